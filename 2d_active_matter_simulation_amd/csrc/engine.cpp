@@ -1,0 +1,286 @@
+// Ensemble engine: many resident runs of mixed size and parameters on one
+// device.  Runs of the same (precision, Nx, Ny) form a size class; one step of
+// a class is the same 8 launches as a single run, each launch covering every
+// run of the class (blockIdx -> (run, tile) through an array of per-run
+// descriptors).  Classes are independent, so each gets its own stream and the
+// step of all classes is captured as one hipGraph that forks into a branch per
+// class and joins at the end: different sizes execute concurrently, no block
+// ever waits on another, and each run's arithmetic is exactly that of the run
+// stepped alone (same kernels, same per-run data, no cross-run reductions).
+//
+// Admission is by an HBM budget on the bytes of run state the engine holds.
+// No reference counterpart: the reference runs one grid per process.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "run_impl.hpp"
+
+using namespace af;
+
+namespace {
+
+struct Slot {
+  bool used = false;
+  af_params p;
+  Geometry g;
+  unsigned char* base = nullptr;
+  long long steps = 0;
+};
+
+struct Class {
+  hipStream_t stream = nullptr;
+  void* descs = nullptr;   // device array of per-run descriptors
+  int cap = 0;             // capacity of descs (entries)
+  size_t desc_bytes = 0;
+};
+
+using ClassKey = std::tuple<int, int, int>;  // precision, nx, ny
+
+}  // namespace
+
+struct af_engine {
+  size_t budget = 0, in_use = 0;
+  std::vector<Slot> slots;
+  std::map<ClassKey, Class> classes;
+  hipStream_t stream = nullptr;
+  // cached graph for the last stepped slot list
+  std::vector<int> graph_slots;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<hipEvent_t> events;
+};
+
+static void drop_graph(af_engine* e) {
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->gexec = nullptr;
+  e->graph = nullptr;
+  e->graph_slots.clear();
+}
+
+static ClassKey key_of(const af_params& p) { return ClassKey(p.precision, p.nx, p.ny); }
+
+extern "C" {
+
+af_status af_engine_create(size_t hbm_budget, af_engine** out) {
+  if (!out) return fail(AF_EINVAL, "out is NULL");
+  af_engine* e = new af_engine();
+  e->budget = hbm_budget;
+  hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) { delete e; return hip_fail(he, "engine stream"); }
+  *out = e;
+  return AF_OK;
+}
+
+af_status af_engine_add(af_engine* e, const af_params* p, const double* omega_hat, const double M[2], int* slot) {
+  if (!e || !p || !omega_hat || !M || !slot) return fail(AF_EINVAL, "NULL argument");
+  af_status st = validate(p);
+  if (st != AF_OK) return st;
+  Geometry g = geometry(p);
+  if (!g.rc || !g.cc || !g.pc) return fail(AF_EUNSUPPORTED, "no kernel for this size");
+  if (e->in_use + g.bytes > e->budget) return fail(AF_ENOMEM, "HBM budget exceeded");
+  unsigned char* base = nullptr;
+  hipError_t he = hipMalloc(&base, g.bytes);
+  if (he != hipSuccess) return fail(AF_ENOMEM, std::string("device allocation: ") + hipGetErrorString(he));
+  st = init_device_block(p, g, base, e->stream);
+  if (st == AF_OK) st = load_state(g, base, omega_hat, M, e->stream);
+  if (st != AF_OK) { (void)hipFree(base); return st; }
+  int id = -1;
+  for (size_t i = 0; i < e->slots.size(); ++i)
+    if (!e->slots[i].used) { id = (int)i; break; }
+  if (id < 0) { id = (int)e->slots.size(); e->slots.emplace_back(); }
+  Slot& s = e->slots[id];
+  s.used = true;
+  s.p = *p;
+  s.g = g;
+  s.base = base;
+  s.steps = 0;
+  e->in_use += g.bytes;
+  Class& c = e->classes[key_of(*p)];
+  if (!c.stream) {
+    he = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+    if (he != hipSuccess) return hip_fail(he, "class stream");
+  }
+  *slot = id;
+  return AF_OK;
+}
+
+af_status af_engine_remove(af_engine* e, int slot) {
+  if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used) return fail(AF_EINVAL, "bad slot");
+  (void)hipStreamSynchronize(e->stream);
+  for (auto& kv : e->classes) (void)hipStreamSynchronize(kv.second.stream);
+  drop_graph(e);
+  Slot& s = e->slots[slot];
+  (void)hipFree(s.base);
+  e->in_use -= s.g.bytes;
+  s = Slot();
+  return AF_OK;
+}
+
+static af_status build_descs(af_engine* e, const std::vector<int>& list,
+                             std::vector<std::pair<ClassKey, std::vector<int>>>& groups) {
+  std::map<ClassKey, std::vector<int>> by;
+  for (int id : list) by[key_of(e->slots[id].p)].push_back(id);
+  groups.assign(by.begin(), by.end());
+  for (auto& gr : groups) {
+    Class& c = e->classes[gr.first];
+    const Geometry& g = e->slots[gr.second[0]].g;
+    const size_t db = g.prec == 64 ? sizeof(InstDesc<double>) : sizeof(InstDesc<float>);
+    if ((int)gr.second.size() > c.cap) {
+      if (c.descs) (void)hipFree(c.descs);
+      c.cap = std::max<int>((int)gr.second.size(), 2 * c.cap);
+      hipError_t he = hipMalloc(&c.descs, (size_t)c.cap * db);
+      if (he != hipSuccess) { c.cap = 0; c.descs = nullptr; return fail(AF_ENOMEM, "descriptor array"); }
+    }
+    c.desc_bytes = db;
+    for (size_t i = 0; i < gr.second.size(); ++i) {
+      const Slot& s = e->slots[gr.second[i]];
+      hipError_t he = hipMemcpyAsync((unsigned char*)c.descs + i * db, s.base + s.g.off_desc, db,
+                                     hipMemcpyDeviceToDevice, e->stream);
+      if (he != hipSuccess) return hip_fail(he, "descriptor copy");
+    }
+  }
+  hipError_t he = hipStreamSynchronize(e->stream);
+  if (he != hipSuccess) return hip_fail(he, "descriptor sync");
+  return AF_OK;
+}
+
+static hipError_t enqueue_step(af_engine* e, const std::vector<std::pair<ClassKey, std::vector<int>>>& groups) {
+  // fork: every class stream waits on the main stream, joins back at the end
+  if (e->events.size() < groups.size() + 1) {
+    for (size_t i = e->events.size(); i < groups.size() + 1; ++i) {
+      hipEvent_t ev;
+      hipError_t he = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (he != hipSuccess) return he;
+      e->events.push_back(ev);
+    }
+  }
+  hipError_t he = hipEventRecord(e->events[0], e->stream);
+  if (he != hipSuccess) return he;
+  for (size_t i = 0; i < groups.size(); ++i) {
+    Class& c = e->classes[groups[i].first];
+    const Slot& s0 = e->slots[groups[i].second[0]];
+    if ((he = hipStreamWaitEvent(c.stream, e->events[0], 0)) != hipSuccess) return he;
+    if ((he = launch_step(s0.g, c.descs, (int)groups[i].second.size(), s0.base + s0.g.off_pool, c.stream)) !=
+        hipSuccess)
+      return he;
+    if ((he = hipEventRecord(e->events[i + 1], c.stream)) != hipSuccess) return he;
+    if ((he = hipStreamWaitEvent(e->stream, e->events[i + 1], 0)) != hipSuccess) return he;
+  }
+  return hipSuccess;
+}
+
+af_status af_engine_step_slots(af_engine* e, const int* slots, int nslots, int nsteps) {
+  if (!e || (nslots > 0 && !slots) || nsteps < 0) return fail(AF_EINVAL, "bad arguments");
+  std::vector<int> list(slots, slots + nslots);
+  std::sort(list.begin(), list.end());
+  list.erase(std::unique(list.begin(), list.end()), list.end());
+  for (int id : list)
+    if (id < 0 || id >= (int)e->slots.size() || !e->slots[id].used) return fail(AF_EINVAL, "bad slot in list");
+  if (list.empty() || nsteps == 0) return AF_OK;
+  std::vector<std::pair<ClassKey, std::vector<int>>> groups;
+  if (list != e->graph_slots || !e->gexec) {
+    drop_graph(e);
+    af_status st = build_descs(e, list, groups);
+    if (st != AF_OK) return st;
+    hipError_t he = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+    if (he == hipSuccess) {
+      hipError_t le = enqueue_step(e, groups);
+      he = hipStreamEndCapture(e->stream, &e->graph);
+      if (le != hipSuccess) he = le;
+      if (he == hipSuccess) he = hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0);
+    }
+    if (he != hipSuccess) {
+      (void)hipGetLastError();
+      drop_graph(e);
+      return hip_fail(he, "engine graph");
+    }
+    e->graph_slots = list;
+  }
+  int done = 0;
+  while (done < nsteps) {
+    const int k = std::min(nsteps - done, kRingCap);
+    for (int i = 0; i < k; ++i) {
+      hipError_t he = hipGraphLaunch(e->gexec, e->stream);
+      if (he != hipSuccess) return hip_fail(he, "engine step");
+    }
+    hipError_t he = hipStreamSynchronize(e->stream);
+    if (he != hipSuccess) return hip_fail(he, "engine sync");
+    for (int id : list) e->slots[id].steps += k;
+    done += k;
+  }
+  // NaN watchdog per run
+  for (int id : list) {
+    MeanState ms;
+    const Slot& s = e->slots[id];
+    hipError_t he = hipMemcpy(&ms, s.base + s.g.off_mean, sizeof(ms), hipMemcpyDeviceToHost);
+    if (he != hipSuccess) return hip_fail(he, "mean download");
+    if (ms.nan_step >= 0)
+      return fail(AF_ENAN, "slot " + std::to_string(id) + ": array contains nan values after " +
+                               std::to_string(ms.nan_step) + " timesteps");
+  }
+  return AF_OK;
+}
+
+af_status af_engine_step(af_engine* e, int nsteps) {
+  if (!e) return fail(AF_EINVAL, "engine is NULL");
+  std::vector<int> all;
+  for (size_t i = 0; i < e->slots.size(); ++i)
+    if (e->slots[i].used) all.push_back((int)i);
+  return af_engine_step_slots(e, all.data(), (int)all.size(), nsteps);
+}
+
+af_status af_engine_get_state(af_engine* e, int slot, double* omega_hat, double M[2]) {
+  if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used || !omega_hat || !M)
+    return fail(AF_EINVAL, "bad arguments");
+  const Slot& s = e->slots[slot];
+  return store_state(s.g, s.base, omega_hat, M, e->stream);
+}
+
+af_status af_engine_get_energy(af_engine* e, int slot, int n, double* energy) {
+  if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used || n < 0 || (n > 0 && !energy))
+    return fail(AF_EINVAL, "bad arguments");
+  const Slot& s = e->slots[slot];
+  if (n > s.steps || n > kRingCap) return fail(AF_EINVAL, "fewer energies available than requested");
+  std::vector<double> ring(kRingCap);
+  hipError_t he = hipMemcpy(ring.data(), s.base + s.g.off_energy, kRingCap * sizeof(double), hipMemcpyDeviceToHost);
+  if (he != hipSuccess) return hip_fail(he, "energy download");
+  for (int i = 0; i < n; ++i) energy[i] = ring[(size_t)((s.steps - n + i) % kRingCap)];
+  return AF_OK;
+}
+
+long long af_engine_steps_done(const af_engine* e, int slot) {
+  if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used) return -1;
+  return e->slots[slot].steps;
+}
+
+size_t af_engine_bytes_in_use(const af_engine* e) { return e ? e->in_use : 0; }
+
+int af_engine_count(const af_engine* e) {
+  if (!e) return 0;
+  int n = 0;
+  for (const auto& s : e->slots) n += s.used ? 1 : 0;
+  return n;
+}
+
+void af_engine_destroy(af_engine* e) {
+  if (!e) return;
+  (void)hipStreamSynchronize(e->stream);
+  for (auto& kv : e->classes) (void)hipStreamSynchronize(kv.second.stream);
+  drop_graph(e);
+  for (auto& s : e->slots)
+    if (s.used) (void)hipFree(s.base);
+  for (auto& kv : e->classes) {
+    if (kv.second.descs) (void)hipFree(kv.second.descs);
+    if (kv.second.stream) (void)hipStreamDestroy(kv.second.stream);
+  }
+  for (auto ev : e->events) (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+}  // extern "C"

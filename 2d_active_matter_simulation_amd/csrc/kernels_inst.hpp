@@ -1,0 +1,38 @@
+// Instantiation helpers shared by kernels_f32.hip and kernels_f64.hip.
+#pragma once
+#include "registry.hpp"
+#include "stage_kernels.hpp"
+
+namespace af {
+
+template <class T, int NX2, int RB, int R, int BX>
+hipError_t launch_row(const void* descs, int ninst, int tiles, const void* pool, hipStream_t s) {
+  hipLaunchKernelGGL((row_pass<T, NX2, RB, R, BX>), dim3(ninst * tiles), dim3(RB * (NX2 / R)), 0, s,
+                     (const InstDesc<T>*)descs, tiles, (const cx<T>*)pool);
+  return hipGetLastError();
+}
+
+template <class T, int NY, int W, int R, int BY>
+hipError_t launch_col(const void* descs, int ninst, int tiles, int stage, const void* pool, hipStream_t s) {
+  hipLaunchKernelGGL((col_pass<T, NY, W, R, BY>), dim3(ninst * tiles), dim3(W * (NY / R)), 0, s,
+                     (const InstDesc<T>*)descs, tiles, stage, (const cx<T>*)pool);
+  return hipGetLastError();
+}
+
+inline int conv_blocks(size_t n) { return (int)((n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192); }
+
+template <class T>
+hipError_t launch_to_bricks(const void* src, void* dst, int nx, int ny, int by, int bx, hipStream_t s) {
+  hipLaunchKernelGGL((to_bricks<T>), dim3(conv_blocks((size_t)ny * (nx / 2))), dim3(256), 0, s,
+                     (const double*)src, (cx<T>*)dst, nx, ny, by, bx);
+  return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_from_bricks(const void* src, void* dst, int nx, int ny, int by, int bx, hipStream_t s) {
+  hipLaunchKernelGGL((from_bricks<T>), dim3(conv_blocks((size_t)ny * (nx / 2 + 1))), dim3(256), 0, s,
+                     (const cx<T>*)src, (double*)dst, nx, ny, by, bx);
+  return hipGetLastError();
+}
+
+}  // namespace af

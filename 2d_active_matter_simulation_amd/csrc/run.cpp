@@ -1,0 +1,439 @@
+// Host stepper behind the C ABI: one af_run = one grid resident on one HIP
+// device, driven on its own stream.  Replaces the setup and time loop of
+// ActFlow_V3-2.cu:409-862 (managed buffers, cuFFT plans, 29 host syncs per
+// step) with one device allocation, a captured hipGraph of the 8 launches of a
+// step, and one synchronisation per chunk of steps.
+#include "run_impl.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace af {
+
+thread_local std::string g_last_error;
+
+af_status fail(af_status s, const std::string& msg) {
+  g_last_error = msg;
+  return s;
+}
+
+af_status hip_fail(hipError_t e, const char* what) {
+  g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return AF_EDEVICE;
+}
+
+static bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+
+af_status validate(const af_params* p) {
+  if (!p) return fail(AF_EINVAL, "params is NULL");
+  if (!pow2_in(p->nx, 32, 4096) || !pow2_in(p->ny, 32, 4096))
+    return fail(AF_EUNSUPPORTED, "nx and ny must be powers of two in [32, 4096]");
+  if (p->precision != 32 && p->precision != 64) return fail(AF_EINVAL, "precision must be 32 or 64");
+  if (p->dealias != 0 && p->dealias != 1)
+    return fail(AF_EUNSUPPORTED, "dealias must be 0 (isotropic 2/3) or 1 (square 2/3) on this backend");
+  if (p->mean_scheme != 0 && p->mean_scheme != 1) return fail(AF_EINVAL, "mean_scheme must be 0 or 1");
+  if (!(p->lx > 0) || !(p->ly > 0) || !std::isfinite(p->lx) || !std::isfinite(p->ly))
+    return fail(AF_EINVAL, "box lengths must be positive");
+  if (!std::isfinite(p->dt) || !std::isfinite(p->lambda) || !std::isfinite(p->beta) || !std::isfinite(p->c0) ||
+      !std::isfinite(p->c2) || !std::isfinite(p->c4))
+    return fail(AF_EINVAL, "non-finite parameter");
+  return AF_OK;
+}
+
+// exp(-2 pi i m / L) in long double with the angle reduced to [0, pi/4] by
+// the symmetries of the unit circle, then rounded once.
+static void unit_root(long m, long L, long double* c, long double* s) {
+  const long double pi = 3.141592653589793238462643383279502884L;
+  m %= L;
+  // 8 m / L in [0, 8): octant o, offset f in [0, 1)
+  const long num = 8 * m;
+  const long o = num / L;
+  const long double f = (long double)(num - o * L) / (long double)L;  // fraction of an octant
+  const long double a = f * pi / 4;                                   // in [0, pi/4)
+  const long double ca = cosl(a), sa = sinl(a), cb = cosl(pi / 4 - a), sb = sinl(pi / 4 - a);
+  long double cc, ss;  // cos, sin of theta = 2 pi m / L = o*pi/4 + a
+  switch (o) {
+    case 0: cc = ca; ss = sa; break;
+    case 1: cc = sb; ss = cb; break;       // pi/2 - (pi/4 - a)
+    case 2: cc = -sa; ss = ca; break;      // pi/2 + a
+    case 3: cc = -cb; ss = sb; break;      // pi - (pi/4 - a)
+    case 4: cc = -ca; ss = -sa; break;     // pi + a
+    case 5: cc = -sb; ss = -cb; break;     // 3pi/2 - (pi/4 - a)
+    case 6: cc = sa; ss = -ca; break;      // 3pi/2 + a
+    default: cc = cb; ss = -sb; break;     // 2pi - (pi/4 - a)
+  }
+  *c = cc;
+  *s = -ss;
+}
+
+template <class T>
+static void build_pool(std::vector<unsigned char>& out, int lmax) {
+  std::vector<cx<T>> v((size_t)2 * lmax, cx<T>{0, 0});
+  for (int L = 1; L <= lmax; L *= 2)
+    for (int m = 0; m < L; ++m) {
+      long double c, s;
+      unit_root(m, L, &c, &s);
+      v[(size_t)L + m] = cx<T>{(T)c, (T)s};
+    }
+  out.resize(v.size() * sizeof(cx<T>));
+  std::memcpy(out.data(), v.data(), out.size());
+}
+
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+Geometry geometry(const af_params* p) {
+  Geometry g;
+  g.prec = p->precision;
+  g.nx2 = p->nx / 2;
+  g.ny = p->ny;
+  g.by = brick_by(g.prec, g.nx2);
+  g.bx = brick_bx(g.prec, g.ny);
+  g.rc = find_row_cfg(g.prec, g.nx2, g.bx);
+  g.cc = find_col_cfg(g.prec, g.ny, g.by);
+  g.pc = find_prec_cfg(g.prec);
+  g.esz = g.pc ? g.pc->elem_bytes : 0;
+  g.nfield = (size_t)g.ny * g.nx2;
+  g.row_tiles = g.rc ? g.ny / g.rc->rb : 0;
+  g.col_tiles = g.cc ? g.nx2 / g.cc->w : 0;
+  g.lmax = p->nx > p->ny ? p->nx : p->ny;
+  size_t off = 0;
+  for (int i = 0; i < 7; ++i) { g.off_field[i] = off; off += align256(g.nfield * g.esz); }
+  g.off_partials = off; off += align256((size_t)g.row_tiles * 4 * sizeof(double));
+  g.off_mean = off; off += align256(sizeof(MeanState));
+  g.off_energy = off; off += align256((size_t)kRingCap * sizeof(double));
+  g.off_desc = off; off += align256(sizeof(InstDesc<double>));
+  g.off_pool = off; off += align256((size_t)2 * g.lmax * g.esz);
+  g.bytes = off;
+  return g;
+}
+
+template <class T>
+static InstDesc<T> make_desc(const af_params* p, const Geometry& g, unsigned char* base) {
+  InstDesc<T> d;
+  std::memset(&d, 0, sizeof(d));
+  cx<T>** f[7] = {&d.wo, &d.acc, &d.um, &d.vm, &d.wm, &d.pm, &d.qm};
+  for (int i = 0; i < 7; ++i) *f[i] = reinterpret_cast<cx<T>*>(base + g.off_field[i]);
+  d.partials = reinterpret_cast<double*>(base + g.off_partials);
+  d.mean = reinterpret_cast<MeanState*>(base + g.off_mean);
+  d.energy = reinterpret_cast<double*>(base + g.off_energy);
+  d.ring_cap = kRingCap;
+  d.nx = p->nx;
+  d.ny = p->ny;
+  d.mask_mode = p->dealias;
+  d.mean_scheme = p->mean_scheme;
+  d.row_tiles = g.row_tiles;
+  const double twopi = 6.283185307179586476925286766559;
+  d.dt = p->dt;
+  d.tkx = twopi / p->lx;
+  d.tky = twopi / p->ly;
+  d.c0 = p->c0;
+  d.c2 = p->c2;
+  d.c4 = p->c4;
+  d.lam = p->lambda;
+  d.beta = p->beta;
+  d.invN = 1.0 / ((double)p->nx * (double)p->ny);
+  d.e = std::exp(p->c0 * p->dt);
+  d.e2 = std::exp(p->c0 * p->dt * 0.5);
+  return d;
+}
+
+// Writes the descriptor, twiddle pool and a fresh mean state into the run's
+// allocation (synchronously on `stream`).
+af_status init_device_block(const af_params* p, const Geometry& g, unsigned char* base, hipStream_t stream) {
+  std::vector<unsigned char> pool;
+  if (g.prec == 64) build_pool<double>(pool, g.lmax); else build_pool<float>(pool, g.lmax);
+  hipError_t e = hipMemcpyAsync(base + g.off_pool, pool.data(), pool.size(), hipMemcpyHostToDevice, stream);
+  if (e != hipSuccess) return hip_fail(e, "upload twiddles");
+  if (g.prec == 64) {
+    InstDesc<double> d = make_desc<double>(p, g, base);
+    e = hipMemcpyAsync(base + g.off_desc, &d, sizeof(d), hipMemcpyHostToDevice, stream);
+  } else {
+    InstDesc<float> d = make_desc<float>(p, g, base);
+    e = hipMemcpyAsync(base + g.off_desc, &d, sizeof(d), hipMemcpyHostToDevice, stream);
+  }
+  if (e != hipSuccess) return hip_fail(e, "upload descriptor");
+  MeanState ms;
+  std::memset(&ms, 0, sizeof(ms));
+  ms.nan_step = -1;
+  e = hipMemcpyAsync(base + g.off_mean, &ms, sizeof(ms), hipMemcpyHostToDevice, stream);
+  if (e != hipSuccess) return hip_fail(e, "upload mean state");
+  e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return hip_fail(e, "init sync");
+  return AF_OK;
+}
+
+// Reference-layout state -> bricks + mean, then the stage-0 column pass.
+af_status load_state(const Geometry& g, unsigned char* base, const double* omega_hat, const double M[2],
+                     hipStream_t stream) {
+  const size_t n = (size_t)g.ny * (g.nx2 + 1) * 2;
+  double* staging = nullptr;
+  hipError_t e = hipMalloc(&staging, n * sizeof(double));
+  if (e != hipSuccess) return fail(AF_ENOMEM, std::string("staging: ") + hipGetErrorString(e));
+  af_status st = AF_OK;
+  MeanState ms;
+  std::memset(&ms, 0, sizeof(ms));
+  ms.M[0] = M[0];
+  ms.M[1] = M[1];
+  ms.Mo[0] = M[0];
+  ms.Mo[1] = M[1];
+  ms.nan_step = -1;
+  do {
+    if ((e = hipMemcpyAsync(staging, omega_hat, n * sizeof(double), hipMemcpyHostToDevice, stream)) != hipSuccess) {
+      st = hip_fail(e, "upload state"); break;
+    }
+    if ((e = g.pc->to_bricks(staging, base + g.off_field[0], g.nx2 * 2, g.ny, g.by, g.bx, stream)) != hipSuccess) {
+      st = hip_fail(e, "to_bricks"); break;
+    }
+    if ((e = hipMemcpyAsync(base + g.off_mean, &ms, sizeof(ms), hipMemcpyHostToDevice, stream)) != hipSuccess) {
+      st = hip_fail(e, "upload mean"); break;
+    }
+    if ((e = g.cc->launch(base + g.off_desc, 1, g.col_tiles, 0, base + g.off_pool, stream)) != hipSuccess) {
+      st = hip_fail(e, "initial column pass"); break;
+    }
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) { st = hip_fail(e, "set_state sync"); break; }
+  } while (0);
+  (void)hipFree(staging);
+  return st;
+}
+
+af_status store_state(const Geometry& g, unsigned char* base, double* omega_hat, double M[2], hipStream_t stream) {
+  const size_t n = (size_t)g.ny * (g.nx2 + 1) * 2;
+  double* staging = nullptr;
+  hipError_t e = hipMalloc(&staging, n * sizeof(double));
+  if (e != hipSuccess) return fail(AF_ENOMEM, std::string("staging: ") + hipGetErrorString(e));
+  af_status st = AF_OK;
+  MeanState ms;
+  do {
+    if ((e = g.pc->from_bricks(base + g.off_field[0], staging, g.nx2 * 2, g.ny, g.by, g.bx, stream)) != hipSuccess) {
+      st = hip_fail(e, "from_bricks"); break;
+    }
+    if ((e = hipMemcpyAsync(omega_hat, staging, n * sizeof(double), hipMemcpyDeviceToHost, stream)) != hipSuccess) {
+      st = hip_fail(e, "download state"); break;
+    }
+    if ((e = hipMemcpyAsync(&ms, base + g.off_mean, sizeof(ms), hipMemcpyDeviceToHost, stream)) != hipSuccess) {
+      st = hip_fail(e, "download mean"); break;
+    }
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) { st = hip_fail(e, "get_state sync"); break; }
+    if (M) { M[0] = ms.M[0]; M[1] = ms.M[1]; }
+  } while (0);
+  (void)hipFree(staging);
+  return st;
+}
+
+// The 8 launches of one step: row pass and column pass for stages 1..4.
+hipError_t launch_step(const Geometry& g, const void* descs, int ninst, const void* pool, hipStream_t s) {
+  for (int stage = 1; stage <= 4; ++stage) {
+    hipError_t e = g.rc->launch(descs, ninst, g.row_tiles, pool, s);
+    if (e != hipSuccess) return e;
+    e = g.cc->launch(descs, ninst, g.col_tiles, stage, pool, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace af
+
+using namespace af;
+
+struct af_run {
+  af_params p;
+  Geometry g;
+  hipStream_t stream = nullptr;
+  unsigned char* base = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool use_graph = true;
+  bool graph_failed = false;
+  bool ready = false;
+  long long steps = 0;
+};
+
+extern "C" {
+
+const char* af_status_str(af_status s) {
+  switch (s) {
+    case AF_OK: return "ok";
+    case AF_EINVAL: return "invalid argument";
+    case AF_ENOMEM: return "out of memory";
+    case AF_EDEVICE: return "device error";
+    case AF_ENAN: return "NaN in the solution";
+    case AF_EIO: return "I/O error";
+    case AF_EUNSUPPORTED: return "unsupported on this backend";
+  }
+  return "unknown status";
+}
+
+const char* af_last_error(void) { return g_last_error.c_str(); }
+
+void af_params_reference(af_params* p, int nx, int ny, double lx, double ly, double dt, double lambda, double alpha,
+                         double beta) {
+  std::memset(p, 0, sizeof(*p));
+  p->nx = nx; p->ny = ny; p->lx = lx; p->ly = ly; p->dt = dt;
+  p->lambda = lambda; p->beta = beta;
+  p->c0 = -1.0 - alpha; p->c2 = 2.0; p->c4 = -1.0;
+  p->precision = 64; p->dealias = 0; p->mean_scheme = 0;
+}
+
+void af_params_jw(af_params* p, int nx, int ny, double lx, double ly, double dt, double lambda, double alpha,
+                  double beta, double gamma0, double gamma2) {
+  af_params_reference(p, nx, ny, lx, ly, dt, lambda, alpha, beta);
+  p->c0 = -alpha; p->c2 = -gamma0; p->c4 = -gamma2;
+}
+
+size_t af_run_device_bytes(const af_params* p) {
+  if (validate(p) != AF_OK) return 0;
+  return geometry(p).bytes;
+}
+
+af_status af_run_create(const af_params* p, af_run** out) {
+  if (!out) return fail(AF_EINVAL, "out is NULL");
+  *out = nullptr;
+  af_status st = validate(p);
+  if (st != AF_OK) return st;
+  af_run* r = new af_run();
+  r->p = *p;
+  r->g = geometry(p);
+  if (!r->g.rc || !r->g.cc || !r->g.pc) { delete r; return fail(AF_EUNSUPPORTED, "no kernel for this size"); }
+  hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete r; return hip_fail(e, "stream create"); }
+  e = hipMalloc(&r->base, r->g.bytes);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(r->stream);
+    delete r;
+    return fail(AF_ENOMEM, std::string("device allocation: ") + hipGetErrorString(e));
+  }
+  st = init_device_block(p, r->g, r->base, r->stream);
+  if (st != AF_OK) { af_run_destroy(r); return st; }
+  *out = r;
+  return AF_OK;
+}
+
+af_status af_run_set_state(af_run* r, const double* omega_hat, const double M[2]) {
+  if (!r || !omega_hat || !M) return fail(AF_EINVAL, "NULL argument");
+  af_status st = load_state(r->g, r->base, omega_hat, M, r->stream);
+  r->ready = (st == AF_OK);
+  r->steps = 0;
+  return st;
+}
+
+af_status af_run_get_state(af_run* r, double* omega_hat, double M[2]) {
+  if (!r || !omega_hat || !M) return fail(AF_EINVAL, "NULL argument");
+  if (!r->ready) return fail(AF_EINVAL, "state not set");
+  return store_state(r->g, r->base, omega_hat, M, r->stream);
+}
+
+static hipError_t run_one_step(af_run* r) {
+  const void* desc = r->base + r->g.off_desc;
+  const void* pool = r->base + r->g.off_pool;
+  if (r->use_graph && !r->graph_failed) {
+    if (!r->gexec) {
+      hipError_t e = hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        hipError_t le = launch_step(r->g, desc, 1, pool, r->stream);
+        e = hipStreamEndCapture(r->stream, &r->graph);
+        if (le == hipSuccess && e == hipSuccess) e = hipGraphInstantiate(&r->gexec, r->graph, nullptr, nullptr, 0);
+        else if (e == hipSuccess) e = le;
+      }
+      if (e != hipSuccess || !r->gexec) {
+        (void)hipGetLastError();
+        r->graph_failed = true;
+        if (r->graph) { (void)hipGraphDestroy(r->graph); r->graph = nullptr; }
+        return launch_step(r->g, desc, 1, pool, r->stream);
+      }
+    }
+    return hipGraphLaunch(r->gexec, r->stream);
+  }
+  return launch_step(r->g, desc, 1, pool, r->stream);
+}
+
+af_status af_run_step(af_run* r, int nsteps, double* energy, long long* nan_step) {
+  if (!r) return fail(AF_EINVAL, "run is NULL");
+  if (nsteps < 0) return fail(AF_EINVAL, "nsteps < 0");
+  if (!r->ready) return fail(AF_EINVAL, "state not set");
+  int done = 0;
+  while (done < nsteps) {
+    const int k = (nsteps - done) < kRingCap ? (nsteps - done) : kRingCap;
+    for (int i = 0; i < k; ++i) {
+      hipError_t e = run_one_step(r);
+      if (e != hipSuccess) return hip_fail(e, "step launch");
+    }
+    MeanState ms;
+    hipError_t e = hipMemcpyAsync(&ms, r->base + r->g.off_mean, sizeof(ms), hipMemcpyDeviceToHost, r->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+    if (e != hipSuccess) return hip_fail(e, "step sync");
+    if (energy) {
+      std::vector<double> ring(kRingCap);
+      e = hipMemcpy(ring.data(), r->base + r->g.off_energy, kRingCap * sizeof(double), hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return hip_fail(e, "energy download");
+      for (int i = 0; i < k; ++i) energy[done + i] = ring[(size_t)((r->steps + i) % kRingCap)];
+    }
+    r->steps += k;
+    done += k;
+    if (ms.nan_step >= 0) {
+      if (nan_step) *nan_step = ms.nan_step;
+      return fail(AF_ENAN, "array contains nan values after " + std::to_string(ms.nan_step) + " timesteps");
+    }
+  }
+  return AF_OK;
+}
+
+long long af_run_steps_done(const af_run* r) { return r ? r->steps : -1; }
+
+af_status af_run_use_graph(af_run* r, int on) {
+  if (!r) return fail(AF_EINVAL, "run is NULL");
+  r->use_graph = on != 0;
+  return AF_OK;
+}
+
+af_status af_run_profile(af_run* r, int nsteps, double* row_ms, double* col_ms, double* total_ms) {
+  if (!r || nsteps <= 0) return fail(AF_EINVAL, "bad arguments");
+  if (!r->ready) return fail(AF_EINVAL, "state not set");
+  const int nev = 8 * nsteps + 1;
+  std::vector<hipEvent_t> ev(nev, nullptr);
+  for (auto& x : ev)
+    if (hipEventCreate(&x) != hipSuccess) return fail(AF_EDEVICE, "event create");
+  const void* desc = r->base + r->g.off_desc;
+  const void* pool = r->base + r->g.off_pool;
+  hipError_t e = hipEventRecord(ev[0], r->stream);
+  int k = 1;
+  for (int s = 0; s < nsteps && e == hipSuccess; ++s)
+    for (int stage = 1; stage <= 4 && e == hipSuccess; ++stage) {
+      e = r->g.rc->launch(desc, 1, r->g.row_tiles, pool, r->stream);
+      if (e == hipSuccess) e = hipEventRecord(ev[k++], r->stream);
+      if (e == hipSuccess) e = r->g.cc->launch(desc, 1, r->g.col_tiles, stage, pool, r->stream);
+      if (e == hipSuccess) e = hipEventRecord(ev[k++], r->stream);
+    }
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  double rs = 0, cs = 0;
+  if (e == hipSuccess) {
+    for (int i = 1; i < nev; ++i) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, ev[i - 1], ev[i]);
+      if (i % 2 == 1) rs += ms; else cs += ms;
+    }
+    float tot = 0;
+    (void)hipEventElapsedTime(&tot, ev[0], ev[nev - 1]);
+    if (row_ms) *row_ms = rs / (4.0 * nsteps);
+    if (col_ms) *col_ms = cs / (4.0 * nsteps);
+    if (total_ms) *total_ms = tot;
+    r->steps += nsteps;
+  }
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  if (e != hipSuccess) return hip_fail(e, "profile");
+  return AF_OK;
+}
+
+void af_run_destroy(af_run* r) {
+  if (!r) return;
+  if (r->stream) (void)hipStreamSynchronize(r->stream);
+  if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
+  if (r->graph) (void)hipGraphDestroy(r->graph);
+  if (r->base) (void)hipFree(r->base);
+  if (r->stream) (void)hipStreamDestroy(r->stream);
+  delete r;
+}
+
+}  // extern "C"

@@ -1,0 +1,41 @@
+// Internals shared by the single-run stepper (run.cpp) and the ensemble
+// engine (engine.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/actflow.h"
+#include "layout.hpp"
+#include "registry.hpp"
+
+namespace af {
+
+constexpr int kRingCap = 1024;   // per-run ring of per-step energies
+
+// Sizes, launch configuration and the offsets of every buffer inside a run's
+// single device allocation.
+struct Geometry {
+  int prec = 0, nx2 = 0, ny = 0, by = 0, bx = 0, lmax = 0;
+  const RowCfg* rc = nullptr;
+  const ColCfg* cc = nullptr;
+  const PrecCfg* pc = nullptr;
+  size_t esz = 0, nfield = 0;
+  int row_tiles = 0, col_tiles = 0;
+  size_t off_field[7] = {0, 0, 0, 0, 0, 0, 0};  // wo, acc, um, vm, wm, pm, qm
+  size_t off_partials = 0, off_mean = 0, off_energy = 0, off_desc = 0, off_pool = 0;
+  size_t bytes = 0;
+};
+
+extern thread_local std::string g_last_error;
+af_status fail(af_status s, const std::string& msg);
+af_status hip_fail(hipError_t e, const char* what);
+af_status validate(const af_params* p);
+Geometry geometry(const af_params* p);
+af_status init_device_block(const af_params* p, const Geometry& g, unsigned char* base, hipStream_t stream);
+af_status load_state(const Geometry& g, unsigned char* base, const double* omega_hat, const double M[2],
+                     hipStream_t stream);
+af_status store_state(const Geometry& g, unsigned char* base, double* omega_hat, double M[2], hipStream_t stream);
+hipError_t launch_step(const Geometry& g, const void* descs, int ninst, const void* pool, hipStream_t s);
+
+}  // namespace af

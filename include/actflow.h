@@ -1,0 +1,169 @@
+/*
+ * actflow.h -- C ABI of libactflow.so, the MI355X (gfx950) implementation of
+ * the ActFlow time step (James-Wilczek vorticity equation with the k=0
+ * mean-velocity ODE) from DominikSuchla/2D_active_matter_simulation.
+ *
+ * The reference exposes no in-process API: its step is inline in main()
+ * (ActFlow_V3-2.cu:718-862) with state in globals and managed buffers
+ * (.cu:183-207, 441-495).  Each entry point below names the reference code it
+ * replaces.  No device pointer and no HIP/torch type crosses this boundary;
+ * the caller owns every host array, which set/get copy.
+ *
+ * Conventions shared with the reference:
+ *   - half spectrum of Ny rows x Nxh = Nx/2+1 columns, element c = y*Nxh + x
+ *     (.cu:416-419), interleaved (re, im) doubles here;
+ *   - omega_hat is the UNNORMALISED forward DFT (cuFFT R2C, .cu:507; the
+ *     numpy rfft2 convention, image.py:69);
+ *   - M[2] is the k=0 DFT coefficient of u, i.e. Nx*Ny*<u> (.cu:1053-1054);
+ *   - energy of step m is 1/2<|u|^2> of the state before step m
+ *     (.cu:1118-1129), labelled m*dt in energy.txt (.cu:1275).
+ */
+#ifndef ACTFLOW_H
+#define ACTFLOW_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  AF_OK = 0,
+  AF_EINVAL = 1,       /* bad argument (sizes, precision, mask mode, ...)      */
+  AF_ENOMEM = 2,       /* device or host allocation failed / over HBM budget   */
+  AF_EDEVICE = 3,      /* HIP runtime error                                    */
+  AF_ENAN = 4,         /* Re omega_hat[0] became NaN (check_func, .cu:884-897)  */
+  AF_EIO = 5,          /* file I/O failure                                     */
+  AF_EUNSUPPORTED = 6  /* valid for the reference but not on this backend      */
+} af_status;
+
+/* Run parameters.  sigma(k) = c0 + c2 k^2 + c4 k^4 is the linear operator.
+ * The reference's -(1+lap)^2 - alpha (.cu:1030) is c0 = -1-alpha, c2 = 2,
+ * c4 = -1 (af_params_reference); the James-Wilczek form -alpha + Gamma0 lap
+ * - Gamma2 lap^2 is c0 = -alpha, c2 = -Gamma0, c4 = -Gamma2 (af_params_jw). */
+typedef struct {
+  int nx, ny;             /* powers of two, 32..4096 each                    */
+  double lx, ly;          /* box lengths (.cu:187-188)                        */
+  double dt;              /* time step (.cu:185)                              */
+  double lambda, beta;    /* advection, cubic friction (.cu:199-203)          */
+  double c0, c2, c4;      /* sigma(k) coefficients                            */
+  int precision;          /* 32: FP32 fields/FFTs, FP64 IF factors and ODE;   *
+                           * 64: FP64 throughout                              */
+  int dealias;            /* 0 isotropic 2/3 (default), 1 square 2/3          */
+  int mean_scheme;        /* 0 reference update (.cu:756-821), 1 lawson       */
+} af_params;
+
+typedef struct af_run af_run;
+
+const char* af_status_str(af_status s);
+/* Last error message of this thread (HIP error text or argument check). */
+const char* af_last_error(void);
+
+/* Fill p from the reference's parameters (alpha of .cu:201, c0=-1-alpha,
+ * c2=2, c4=-1), precision 64, isotropic 2/3, reference mean scheme. */
+void af_params_reference(af_params* p, int nx, int ny, double lx, double ly, double dt,
+                         double lambda, double alpha, double beta);
+/* Same with the James-Wilczek coefficients (c0=-alpha, c2=-Gamma0, c4=-Gamma2). */
+void af_params_jw(af_params* p, int nx, int ny, double lx, double ly, double dt,
+                  double lambda, double alpha, double beta, double gamma0, double gamma2);
+
+/* Replaces the setup of .cu:409-521 and 696-705 (derived constants, the 22
+ * cudaMallocManaged buffers, cufftPlan2d x2, wavenumber/cutoff/lin tables).
+ * Allocates the run's device memory on the current HIP device. */
+af_status af_run_create(const af_params* p, af_run** out);
+/* Replaces the state load of .cu:615-634 / the IC hand-off of .cu:660-679.
+ * omega_hat: 2*Ny*(Nx/2+1) doubles (reference layout).  M: 2 doubles.
+ * The kx = Nx/2 column is not represented on this backend (every supported
+ * mask removes it from the nonlinear term); it is taken as zero. */
+af_status af_run_set_state(af_run* r, const double* omega_hat, const double M[2]);
+/* Replaces the device->host copy of save_func (.cu:1259-1260). */
+af_status af_run_get_state(af_run* r, double* omega_hat, double M[2]);
+/* Replaces nsteps iterations of the time loop .cu:718-821 (and check_func,
+ * .cu:850-861).  energy: nsteps doubles or NULL.  Returns AF_ENAN if
+ * Re omega_hat[0] became NaN; *nan_step (if non-NULL) then receives the
+ * 1-based step count since set_state at which it happened. */
+af_status af_run_step(af_run* r, int nsteps, double* energy, long long* nan_step);
+/* Steps completed since set_state. */
+long long af_run_steps_done(const af_run* r);
+void af_run_destroy(af_run* r);
+
+/* Per-kernel timing with HIP events on the run's stream: runs nsteps steps
+ * with an event pair around every launch and returns the mean duration in
+ * milliseconds of the row pass and of the column pass (stages 1-4), and the
+ * wall time of the whole bracketed region.  No reference counterpart
+ * (measurement hook for bench.py). */
+af_status af_run_profile(af_run* r, int nsteps, double* row_ms, double* col_ms, double* total_ms);
+/* Use (1) or bypass (0) a captured hipGraph for one step (default 1). */
+af_status af_run_use_graph(af_run* r, int on);
+/* Bytes of device memory held by a run with these parameters. */
+size_t af_run_device_bytes(const af_params* p);
+
+/* ---- ensemble engine: many resident runs, batched launches ------------- */
+typedef struct af_engine af_engine;
+/* hbm_budget: bytes the engine may hold in run state (admission limit). */
+af_status af_engine_create(size_t hbm_budget, af_engine** out);
+/* Admit a run; fails with AF_ENOMEM if it would exceed the budget.  The
+ * state is copied in; *slot receives its handle. */
+af_status af_engine_add(af_engine* e, const af_params* p, const double* omega_hat,
+                        const double M[2], int* slot);
+af_status af_engine_remove(af_engine* e, int slot);
+/* Advance every resident run by nsteps steps, all runs of one size class in
+ * the same kernel launches.  Per-run results are bitwise identical to stepping
+ * the run alone. */
+af_status af_engine_step(af_engine* e, int nsteps);
+/* Advance only the listed slots (same batching within the list). */
+af_status af_engine_step_slots(af_engine* e, const int* slots, int nslots, int nsteps);
+af_status af_engine_get_state(af_engine* e, int slot, double* omega_hat, double M[2]);
+/* Copies the last n per-step energies of a slot (oldest first). */
+af_status af_engine_get_energy(af_engine* e, int slot, int n, double* energy);
+long long af_engine_steps_done(const af_engine* e, int slot);
+size_t af_engine_bytes_in_use(const af_engine* e);
+int af_engine_count(const af_engine* e);
+void af_engine_destroy(af_engine* e);
+
+/* ---- host formats (reference file contract) --------------------------- */
+typedef struct {
+  double T, dt, Lx, Ly;   /* .cu:184-188 */
+  int Nx, Ny;             /* .cu:189-190 */
+  double lambda, alpha, beta, dt_snap;   /* .cu:199-203, 196 */
+  char initial[256];      /* initial_condition line (.cu:192)  */
+  char directory[1024];   /* save_directory (.cu:194)          */
+  /* extension keys (absent from the reference) */
+  int has_gamma;          /* Gamma0:/Gamma2: given -> JW coefficients */
+  double gamma0, gamma2;
+  int precision, dealias, mean_scheme, snapshot_precision;
+  int M, snap;            /* round(T/dt), round(dt_snap/dt) (.cu:409-420) */
+} af_spec;
+
+/* Defaults of .cu:183-204 (note Ly and Ny are bound at static init). */
+void af_spec_defaults(af_spec* s);
+/* Parse <dir>/specifications.txt with the semantics of .cu:320-408.
+ * err (optional, errlen bytes) receives the reference's message on failure. */
+af_status af_spec_read(const char* path, af_spec* s, char* err, size_t errlen);
+/* Write the example file of the `spec` subcommand (.cu:250-266). */
+af_status af_spec_write_example(const char* path, const af_spec* s);
+/* Snapshot text format of save_func (.cu:1245-1266): "re,im," per element
+ * then "Mx,My", at `precision` significant digits (6 = reference). */
+af_status af_snapshot_write(const char* path, int nx, int ny, const double* omega_hat,
+                            const double M[2], int precision);
+/* Reader with the semantics of .cu:615-634 (>> then ignore up to ','). */
+af_status af_snapshot_read(const char* path, int nx, int ny, double* omega_hat, double M[2]);
+/* One energy line "t E\n" appended (.cu:1268-1278). */
+af_status af_energy_append(const char* path, double t, double E);
+
+/* Band-limited seeded initial condition (no reference counterpart; the
+ * reference's `random` IC is seeded from the clock, .cu:539-547):
+ * c_n = amp * g * exp(i theta) on 0 < |n|^2 <= kmax^2 (lattice indices),
+ * g ~ N(0,1) by Box-Muller, theta ~ U[0,2pi), SplitMix64(seed), Hermitian on
+ * the nx = 0 column, stored as Nx*Ny*c_n (unnormalised DFT).  M = 0. */
+af_status af_ic_bandlimited(int nx, int ny, unsigned long long seed, double amp, double kmax,
+                            double* omega_hat);
+/* The reference's `random [n]` IC (.cu:535-566): U(-1,1) real field from
+ * mt19937_64(seed), R2C on the host, masked, omega_hat[0] = 0.  seed is
+ * explicit here (the reference derives it from the clock). */
+af_status af_ic_random(int nx, int ny, unsigned long long seed, int dealias, double* omega_hat);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
