@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Benchmark of the ActFlow time step on one MI355X.
+
+A "step" is one full IF-RK step (4 stages = 8 fused kernel launches) of the
+James-Wilczek vorticity equation plus the k=0 mean-velocity ODE and the
+per-step energy, on synthetic input (seeded band-limited initial condition)
+already resident in HBM.
+
+Headline workload (`value`): single run, 1024 x 1024 grid, FP32 fields and
+FFTs with FP64 integrating factors and reductions, dt = 1e-4, lambda0 = 3.5,
+alpha = -0.8, beta = 0.01, Gamma0 = -0.045, Gamma2 = 2.5e-5.  The same
+workload in FP64 and at 2048^2 / 4096^2 is reported under "extra".
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-pt-steps/s @1024²/2048², FP32 vs FP64; % HBM roofline; served p50/p99"
+HBM_PEAK_GBS = 8000.0
+PHYS = dict(lx=2 * np.pi, lam=3.5, alpha=-0.8, beta=0.01, gamma0=-0.045, gamma2=2.5e-5)
+
+
+def field_bytes(n, prec):
+    """One stored complex field: Ny x Nx/2 elements (Nyquist column not stored)."""
+    return n * (n // 2) * (8 if prec == 32 else 16)
+
+
+# Compulsory HBM traffic per launch in complex-field transfers (read + write):
+#   row pass: reads v_x, v_y, omega (mixed), writes P, Q             -> 5
+#   column pass: reads P, Q, omega_old (stages 1-3), acc (stages 2-4);
+#                writes acc (or the new state) and v_x, v_y, omega   -> 7, 8, 8, 7
+ROW_FIELDS = 5.0
+COL_FIELDS = (7 + 8 + 8 + 7) / 4.0
+
+
+def read_traffic(workload_tag):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE), from the committed summary profiles/traffic_<tag>.txt."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload_tag}.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except ValueError:
+                pass
+    return vals or None
+
+
+def run_workload(af, n, prec, steps, warmup, seed=1):
+    p = af.params(n, dt=1e-4 if n <= 1024 else 5e-5, precision=prec, **PHYS)
+    run = af.Run(p)
+    run.set_state(af.ic_bandlimited(n, seed=seed, amp=1e-3, kmax=10.0))
+    if warmup:
+        run.step(warmup)
+    t0 = time.perf_counter()
+    e = run.step(steps, energy=True)          # returns after the device finished
+    wall = time.perf_counter() - t0
+    assert np.all(np.isfinite(e))
+    # per-kernel durations: HIP events around every launch on the run's stream
+    kprof = max(10, min(steps, 200))
+    row_ms, col_ms, tot_ms = run.profile(kprof)
+    run.close()
+    return dict(n=n, prec=prec, steps=steps, wall=wall, ms_per_step=wall / steps * 1e3,
+                value=n * n * steps / wall, row_ms=row_ms, col_ms=col_ms, prof_steps=kprof,
+                prof_ms_per_step=tot_ms / kprof)
+
+
+def roofline(r):
+    fb = field_bytes(r["n"], r["prec"])
+    kernels = {"col_pass": (COL_FIELDS * fb, r["col_ms"]), "row_pass": (ROW_FIELDS * fb, r["row_ms"])}
+    name = max(kernels, key=lambda k: kernels[k][1])
+    alg, ms = kernels[name]
+    achieved = alg / (ms * 1e-3) / 1e9
+    return name, alg, ms, achieved
+
+
+def cpu_baseline(n=1024, prec=32, steps=3):
+    """The oracle (C++ restatement of the reference step) on the host cores."""
+    from oracle.oracle_py import Oracle
+    af = importlib.import_module("2d_active_matter_simulation_amd")
+    threads = min(os.cpu_count() or 1, 16)
+    o = Oracle(n, dt=1e-4, lam=PHYS["lam"], beta=PHYS["beta"], c0=-PHYS["alpha"], c2=-PHYS["gamma0"],
+               c4=-PHYS["gamma2"], precision=prec, nthreads=threads)
+    o.set_state(af.ic_bandlimited(n, seed=1, amp=1e-3, kmax=10.0))
+    o.step(1)
+    t0 = time.perf_counter()
+    o.step(steps, energy=True)
+    wall = time.perf_counter() - t0
+    return {"value": n * n * steps / wall, "unit": "grid-pt-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the {n}^2 FP{prec} oracle after 1 warm-up step "
+                      f"({wall:.1f} s, {threads} OpenMP threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip FP64 / 2048^2 / 4096^2 lines")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(instances shard across GPUs without communication; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    af = importlib.import_module("2d_active_matter_simulation_amd")
+    af.lib()
+
+    head = run_workload(af, 1024, 32, args.steps, args.warmup)
+    name, alg, kms, achieved = roofline(head)
+    traffic = read_traffic("1024_fp32")
+    tr_bytes = None
+    if traffic and f"{name}_bytes_per_launch" in traffic:
+        tr_bytes = traffic[f"{name}_bytes_per_launch"]
+    extra = {}
+    if not args.no_extra:
+        for n, prec, steps in ((1024, 64, max(200, args.steps // 4)), (2048, 32, max(100, args.steps // 8)),
+                               (2048, 64, max(50, args.steps // 16)), (4096, 32, max(20, args.steps // 32))):
+            r = run_workload(af, n, prec, steps, min(args.warmup, 20))
+            nm, a, ms, ach = roofline(r)
+            extra[f"{n}_fp{prec}"] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "dominant": nm,
+                                      "dominant_ms": ms, "achieved_GBs": ach,
+                                      "frac": ach / HBM_PEAK_GBS,
+                                      "step_model_frac": 76 * n * (n // 2 + 1) * (8 if prec == 32 else 16)
+                                      / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        extra["fp32_over_fp64_1024"] = head["value"] / extra["1024_fp64"]["value"]
+    out = {
+        "metric": METRIC,
+        "value": head["value"],
+        "unit": "grid-pt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "single run 1024x1024, FP32 fields/FFTs, FP64 IF factors + reductions, dt=1e-4, "
+                               "lambda0=3.5 alpha=-0.8 beta=0.01 Gamma0=-0.045 Gamma2=2.5e-5, band-limited IC "
+                               "(seed 1, amp 1e-3, |k|<=10), per-step energy on",
+                   "grid": [1024, 1024], "instances": 1},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": tr_bytes,
+                     "algorithmic_bytes_per_launch": alg, "avg_launch_ms": kms,
+                     "events_over_steps": head["prof_steps"],
+                     "step_model_frac": 76 * 1024 * 513 * 8 / (head["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "extra": extra,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

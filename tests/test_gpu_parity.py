@@ -1,0 +1,196 @@
+"""GPU parity: the HIP step (through the C ABI) against the CPU oracle on the
+same seeded inputs, plus size-independent properties at full sizes.
+
+Tolerances: FP64 <= 1e-12 relative L2 on omega (real-space norm via Parseval
+weights) after 100 steps; FP32 (FP32 fields and FFTs, FP64 integrating
+factors and reductions) against the FP64 oracle <= 2e-5 after 100 steps
+(measured ~5e-6)."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+JW = dict(lam=3.5, alpha=-0.8, beta=0.01, gamma0=-0.045, gamma2=2.5e-5)
+
+
+def oracle_for(orc, p, nthreads=4):
+    return orc.Oracle.from_params(p, nthreads=nthreads)
+
+
+def compare(af, orc, p, w0, M0, chunks, tol_w, tol_e, nthreads=4):
+    run = af.Run(p)
+    run.set_state(w0, M0)
+    o = oracle_for(orc, p if p.precision == 64 else _as64(af, p), nthreads)
+    o.set_state(w0, M0)
+    worst = 0.0
+    for n in chunks:
+        eg = run.step(n, energy=True)
+        eo = o.step(n, energy=True)
+        wg, mg = run.get_state()
+        wo, mo = o.get_state()
+        err = rel_l2(wg, wo)
+        worst = max(worst, err)
+        assert err <= tol_w, (n, err)
+        assert np.abs(eg - eo).max() <= tol_e * np.abs(eo).max()
+        assert np.allclose(mg, mo, rtol=tol_w * 10, atol=tol_w * 10 * max(1.0, np.abs(mo).max()))
+    run.close()
+    return worst
+
+
+def _as64(af, p):
+    q = af.AfParams.from_buffer_copy(p)
+    q.precision = 64
+    return q
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_parity_fp64_config1(af, orc, seed):
+    """256^2, dt = 2e-4, band-limited IC (amp 1e-3, |k| <= 10), 100 steps."""
+    p = af.params(256, dt=2e-4, precision=64, **JW)
+    w0 = af.ic_bandlimited(256, seed=seed, amp=1e-3, kmax=10.0)
+    compare(af, orc, p, w0, (0.0, 0.0), (1, 9, 90), 1e-12, 1e-12)
+
+
+def test_parity_fp64_developed(af, orc):
+    """O(1) amplitude so the nonlinear and cubic terms are large, with a mean flow."""
+    N = 128
+    p = af.params(N, dt=2e-4, precision=64, **JW)
+    w0 = af.ic_bandlimited(N, seed=3, amp=2.0, kmax=12.0)
+    compare(af, orc, p, w0, (0.4 * N * N, -0.25 * N * N), (1, 99), 1e-12, 1e-12)
+
+
+@pytest.mark.parametrize("nx,ny,dealias,mean_scheme", [(64, 128, 0, 0), (128, 64, 1, 1), (32, 32, 0, 1),
+                                                       (256, 32, 1, 0)])
+def test_parity_fp64_reference_operator(af, orc, nx, ny, dealias, mean_scheme):
+    """The reference's own operator -(1+lap)^2 - alpha (.cu:1030) in a
+    non-2pi box, rectangular grids, both masks and both mean schemes."""
+    p = af.params(nx, ny, lx=2 * np.pi * 4, ly=2 * np.pi * 4 * ny / nx, dt=5e-3, lam=7.0, alpha=-0.8, beta=0.01,
+                  precision=64, dealias=dealias, mean_scheme=mean_scheme)
+    w0 = af.ic_bandlimited(nx, ny, seed=11, amp=0.5, kmax=8.0)
+    compare(af, orc, p, w0, (0.3 * nx * ny, 0.1 * nx * ny), (3, 47), 1e-12, 1e-12)
+
+
+def test_parity_fp32_error_stated(af, orc):
+    p = af.params(256, dt=2e-4, precision=32, **JW)
+    w0 = af.ic_bandlimited(256, seed=1, amp=2.0, kmax=10.0)
+    worst = compare(af, orc, p, w0, (0.0, 0.0), (1, 9, 90), 2e-5, 2e-6)
+    print(f"FP32 vs FP64 oracle, 256^2, 100 steps: rel L2 {worst:.2e}")
+
+
+def test_linear_exactness_gpu(af):
+    """lambda = beta = 0: omega_n = E^n omega_0 (T4)."""
+    N, dt = 64, 1e-2
+    p = af.params(N, dt=dt, lam=0.0, alpha=-0.8, beta=0.0, precision=64)
+    w0 = af.ic_bandlimited(N, seed=5, amp=1.0, kmax=6.0)
+    run = af.Run(p)
+    run.set_state(w0)
+    run.step(7)
+    w, M = run.get_state()
+    kx = np.arange(N // 2 + 1)[None, :]
+    ky = np.fft.fftfreq(N, 1.0 / N)[:, None]
+    k2 = kx ** 2 + ky ** 2
+    E = np.exp((-(1 - k2) ** 2 + 0.8) * dt)
+    assert np.abs(w - w0 * E ** 7).max() <= 1e-13 * np.abs(w0).max()
+
+
+def test_mean_ode_gpu(af, orc):
+    """omega = 0, M != 0: the 2-D machinery reduces to the scalar recurrence."""
+    from test_oracle_physics import mean_recurrence
+    N, beta, dt = 64, 0.05, 1e-2
+    for scheme in (0, 1):
+        p = af.params(N, dt=dt, lam=2.0, alpha=-0.8, beta=beta, precision=64, mean_scheme=scheme)
+        run = af.Run(p)
+        M0 = np.array([0.9, -0.3]) * N * N
+        run.set_state(np.zeros((N, N // 2 + 1), complex), M0)
+        run.step(20)
+        w, M = run.get_state()
+        ref = mean_recurrence(M0, beta, N, dt, np.exp(-0.2 * dt), np.exp(-0.1 * dt), 20, scheme == 1)   # c0 = -1 - alpha
+        assert np.allclose(M, ref, rtol=1e-13)
+        assert np.abs(w).max() < 1e-9 * N * N
+
+
+def test_graph_and_direct_launch_identical(af):
+    p = af.params(128, dt=2e-4, precision=32, **JW)
+    w0 = af.ic_bandlimited(128, seed=4, amp=1.0)
+    out = []
+    for g in (True, False):
+        run = af.Run(p)
+        run.use_graph(g)
+        run.set_state(w0, (10.0, -5.0))
+        e = run.step(25, energy=True)
+        out.append((run.get_state(), e))
+        run.close()
+    (wa, ma), ea = out[0]
+    (wb, mb), eb = out[1]
+    assert np.array_equal(wa, wb) and np.array_equal(ma, mb) and np.array_equal(ea, eb)
+
+
+def test_state_roundtrip_and_nyquist(af):
+    N = 64
+    rng = np.random.default_rng(0)
+    w = rng.standard_normal((N, N // 2 + 1)) + 1j * rng.standard_normal((N, N // 2 + 1))
+    for prec in (64, 32):
+        run = af.Run(af.params(N, precision=prec))
+        run.set_state(w, (1.25, -3.5))
+        back, M = run.get_state()
+        ref = w.copy()
+        ref[:, -1] = 0            # the kx = Nx/2 column is not represented
+        if prec == 64:
+            assert np.array_equal(back, ref)
+        else:
+            assert np.array_equal(back, ref.real.astype(np.float32) + 1j * ref.imag.astype(np.float32))
+        assert np.array_equal(M, [1.25, -3.5])
+
+
+def test_nan_watchdog(af):
+    """A blow-up is reported as AF_ENAN with the step count (check_func,
+    .cu:884-897)."""
+    N = 64
+    p = af.params(N, dt=0.5, lam=50.0, alpha=-50.0, beta=-5.0, precision=32)
+    run = af.Run(p)
+    run.set_state(af.ic_bandlimited(N, seed=1, amp=100.0, kmax=8.0))
+    with pytest.raises(af.ActFlowError) as ei:
+        run.step(200)
+    assert ei.value.status == af.AF_ENAN
+
+
+def _energy_spectral(w, M, N, lx=2 * np.pi):
+    """1/2 <|u|^2> from omega^ by Biot-Savart in spectral space + Parseval."""
+    kx = 2 * np.pi / lx * np.arange(N // 2 + 1)[None, :]
+    ky = 2 * np.pi / lx * np.fft.fftfreq(N, 1.0 / N)[:, None]
+    k2 = kx ** 2 + ky ** 2
+    k2[0, 0] = 1.0
+    u2 = np.abs(w) ** 2 / k2
+    u2[0, 0] = 0.0
+    wt = np.full(w.shape, 2.0)
+    wt[:, 0] = 1
+    wt[:, -1] = 1
+    return 0.5 * np.sum(wt * u2) / (N * N) ** 2 + 0.5 * np.dot(M, M) / (N * N) ** 2
+
+
+@pytest.mark.parametrize("N,prec", [(512, 64), (1024, 32), (2048, 32), (2048, 64), (4096, 32), (4096, 64)])
+def test_full_size_properties(af, N, prec):
+    """At full sizes: w^[0] stays 0, masked modes stay 0, the reported stage-1
+    energy equals 1/2<|u|^2> of the state (Parseval), finite evolution."""
+    p = af.params(N, dt=5e-5, precision=prec, **JW)
+    run = af.Run(p)
+    run.set_state(af.ic_bandlimited(N, seed=1, amp=0.5, kmax=10.0), (0.1 * N * N, 0.0))
+    run.step(5)
+    w, M = run.get_state()
+    e = run.step(1, energy=True)[0]
+    assert w[0, 0] == 0
+    x = np.arange(N // 2 + 1)[None, :]
+    y = np.minimum(np.arange(N), N - np.arange(N))[:, None]
+    assert np.all(w[9 * (x ** 2 + y ** 2) > N * N] == 0)
+    ref = _energy_spectral(w, M, N)
+    assert abs(e - ref) <= (1e-12 if prec == 64 else 2e-6) * ref
+    run.close()
+
+
+def test_parity_fp64_1024_two_steps(af, orc):
+    N = 1024
+    p = af.params(N, dt=1e-4, precision=64, **JW)
+    w0 = af.ic_bandlimited(N, seed=1, amp=2.0, kmax=12.0)
+    compare(af, orc, p, w0, (0.0, 0.0), (2,), 1e-12, 1e-12, nthreads=8)

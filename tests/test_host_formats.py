@@ -1,0 +1,167 @@
+"""Drop-in file contract (no GPU): specifications.txt parsing and the `spec`
+writer (ActFlow_V3-2.cu:250-266, 320-420), snapshot text files
+(.cu:1245-1266, 615-634, image.py:49-52), energy lines (.cu:1268-1278), the
+initial conditions (.cu:535-566, 660-679) and the CLI's error exits."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+CLI = os.path.join(ROOT, "2d_active_matter_simulation_amd", "actflow")
+
+
+def test_spec_known_answer(af, tmp_path):
+    shutil.copy(os.path.join(GOLDEN, "specifications.txt"), tmp_path / "specifications.txt")
+    s = af.spec_read(str(tmp_path))
+    f32 = lambda v: float(np.float32(v))
+    assert s.T == 5000 and s.dt == f32(0.005)
+    assert s.Lx == f32(628.3185307) and s.Ly == f32(628.3185307)
+    assert s.Nx == 1024 and s.Ny == 1024
+    assert s.alpha == f32(-0.8) and s.beta == f32(0.01) and s.lambda_ == 7
+    assert s.dt_snap == 1000
+    assert s.initial.decode() == "random 42"
+    assert s.directory.decode() == "/new_simulation/snapshots_alpha-08_lambda7"
+    # M = round(T/dt), snap = round(dt_snap/dt) in float arithmetic (.cu:409, 420)
+    assert s.M == 1_000_000 and s.snap == 200_000
+
+
+def test_spec_defaults_do_not_follow_lx_nx(af, tmp_path):
+    """Ly{Lx} and Ny{Nx} bind at static initialisation (.cu:188-190)."""
+    (tmp_path / "specifications.txt").write_text("Lx:\n100\nNx:\n256\n")
+    s = af.spec_read(str(tmp_path))
+    assert s.Lx == 100 and s.Ly == 250 and s.Nx == 256 and s.Ny == 1024
+    assert s.directory.decode() == str(tmp_path)
+
+
+def test_spec_unknown_key_and_blank_line(af, tmp_path):
+    for text in ("T:\n10\nfoo:\n1\n", "T:\n10\n\n"):
+        (tmp_path / "specifications.txt").write_text(text)
+        with pytest.raises(af.ActFlowError) as ei:
+            af.spec_read(str(tmp_path))
+        assert "unknown specification" in str(ei.value)
+
+
+def test_spec_extension_keys(af, tmp_path):
+    (tmp_path / "specifications.txt").write_text(
+        "Gamma0:\n-0.045\nGamma2:\n2.5e-5\nprecision:\n64\ndealias:\n1\nmean_scheme:\n1\nsnapshot_precision:\n17\n")
+    s = af.spec_read(str(tmp_path))
+    assert s.has_gamma == 1 and np.isclose(s.gamma0, -0.045) and np.isclose(s.gamma2, 2.5e-5)
+    assert (s.precision, s.dealias, s.mean_scheme, s.snapshot_precision) == (64, 1, 1, 17)
+
+
+def test_spec_writer_bytes(tmp_path):
+    """`spec` writes the defaults in the order of .cu:259-264, no final newline."""
+    r = subprocess.run([CLI, "spec"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 0 and "example has been created" in r.stdout
+    expect = ("T:\n100\ndt:\n0.002\nLx:\n250\nLy:\n250\nNx:\n1024\nNy:\n1024\nlambda:\n7\nalpha:\n-0.8\n"
+              "beta:\n0.01\ndt_snap:\n0.5\ninitial_condition:\nrandom\nsave_directory:\n./")
+    assert (tmp_path / "specifications.txt").read_text() == expect
+
+
+def test_snapshot_layout_image_py_rule(af, tmp_path):
+    """Read back with image.py's rule: fromfile(sep=','), drop the last 2,
+    reshape (Ny, Nx+2), view complex128."""
+    rng = np.random.default_rng(0)
+    ny, nx = 8, 16
+    w = rng.standard_normal((ny, nx // 2 + 1)) + 1j * rng.standard_normal((ny, nx // 2 + 1))
+    M = np.array([1.5, -2.25])
+    path = tmp_path / "ActTurb_snap3.bin"
+    af.snapshot_write(path, w, M, precision=17)
+    arr = np.fromfile(path, sep=",")
+    assert np.array_equal(arr[-2:], M)
+    back = arr[:-2].reshape((ny, nx + 2)).view(np.complex128)
+    assert np.array_equal(back, w)
+    text = path.read_text()
+    assert not text.endswith(",") and not text.endswith("\n")
+    w2, M2 = af.snapshot_read(path, nx, ny)
+    assert np.array_equal(w2, w) and np.array_equal(M2, M)
+
+
+def test_snapshot_default_precision_is_ostream(af, tmp_path):
+    """Default `<<` formatting: %g with 6 significant digits (.cu:1261-1263)."""
+    w = np.array([[1234567.0 + 0.000123456789j, -0.5 + 0j, 3.0 + 1e-20j]])
+    path = tmp_path / "s.bin"
+    af.snapshot_write(path, w.reshape(1, 3), (1e6, -0.0), precision=6)
+    assert path.read_text() == "1.23457e+06,0.000123457,-0.5,0,3,1e-20,1e+06,-0"
+    w2, M2 = af.snapshot_read(path, 4, 1)
+    assert np.allclose(w2, [[1.23457e6 + 0.000123457j, -0.5, 3 + 1e-20j]])
+
+
+def test_energy_line_format(af, tmp_path):
+    path = tmp_path / "energy.txt"
+    af.energy_append(path, 0.002, 1.0 / 3.0)
+    af.energy_append(path, 1000.0, 2.5e-7)
+    assert path.read_text() == "0.002 0.333333\n1000 2.5e-07\n"
+
+
+def test_bandlimited_ic_fixture(af):
+    d = np.load(os.path.join(GOLDEN, "bandlimited_ic.npz"))
+    for seed, key, amp, kmax, N in ((1, "1", 1e-3, 10.0, 256), (2, "2", 1e-3, 10.0, 64),
+                                    (12345, "12345", 1e-3, 10.0, 128), (7, "7_big", 2.5, 4.5, 32)):
+        w = af.ic_bandlimited(N, seed=seed, amp=amp, kmax=kmax)
+        modes, vals = d[f"modes_{key}"], d[f"vals_{key}"]
+        got = np.array([w[j % N, i] for i, j in modes]) / (N * N)
+        assert np.allclose(got, vals, rtol=1e-14, atol=0)
+        # Hermitian nx = 0 column, zero mean, nothing outside the band
+        col = w[:, 0]
+        assert np.allclose(col[1:], np.conj(col[1:][::-1]))
+        assert w[0, 0] == 0
+        nz = np.count_nonzero(w)
+        assert nz == len(modes) + np.sum(modes[:, 0] == 0)
+    assert len(d["modes_1"]) == 158   # 316 lattice modes with 0 < |n|^2 <= 100
+
+
+def test_random_ic_properties(af):
+    N = 64
+    w = af.ic_random(N, seed=42, dealias=0)
+    assert w[0, 0] == 0
+    w2 = af.ic_random(N, seed=42, dealias=0)
+    assert np.array_equal(w, w2)
+    assert not np.array_equal(w, af.ic_random(N, seed=43, dealias=0))
+    x = np.arange(N // 2 + 1)[None, :]
+    y = np.minimum(np.arange(N), N - np.arange(N))[:, None]
+    outside = 9 * (x ** 2 + y ** 2) > N * N
+    assert np.all(w[outside] == 0)
+    wf = af.ic_random(N, seed=7, dealias=1)
+    inside = (3 * x <= N) & (3 * y <= N)
+    assert np.all(wf[~inside] == 0)
+    # both masks transform the same U(-1,1) field: equal where both keep
+    a = af.ic_random(N, seed=7, dealias=0)
+    both = (a != 0) & (wf != 0)
+    assert both.sum() > 500 and np.array_equal(a[both], wf[both])
+    # iid U(-1,1) (variance 1/3): E|X_k|^2 = N^2 / 3 per coefficient
+    p = np.mean(np.abs(a[a != 0]) ** 2) / (N * N)
+    assert 0.28 < p < 0.39
+
+
+def run_cli(args, cwd):
+    return subprocess.run([CLI] + args, cwd=cwd, capture_output=True, text=True, timeout=60)
+
+
+def test_cli_argument_errors(tmp_path):
+    r = run_cli([], tmp_path)
+    assert r.returncode == 1 and "More than one argument is no longer supported" in r.stderr
+    r = run_cli(["a", "b"], tmp_path)
+    assert r.returncode == 1
+    r = run_cli(["help"], tmp_path)
+    assert r.returncode == 0 and "specifications.txt" in r.stdout
+
+
+def test_cli_failure_injection(tmp_path):
+    d = tmp_path / "sim"
+    d.mkdir()
+    r = run_cli([str(d)], tmp_path)
+    assert r.returncode == 1 and 'Error: Unable to open "specifications.txt".' in r.stderr
+    (d / "specifications.txt").write_text("T:\n1\nbogus:\n2\n")
+    r = run_cli([str(d)], tmp_path)
+    assert r.returncode == 1 and "unknown specification: bogus:" in r.stderr
+    (d / "specifications.txt").write_text("T:\n1\ndt:\n0.1\nNx:\n64\nNy:\n64\ninitial_condition:\nsmooth\n")
+    r = run_cli([str(d)], tmp_path)
+    assert r.returncode == 1 and "Error: unknown specification for the initial condition" in r.stderr
+    (d / "specifications.txt").write_text("T:\n1\ndt:\n0.1\nNx:\n64\nNy:\n64\ninitial_condition:\ncontinue\n")
+    r = run_cli([str(d)], tmp_path)
+    assert r.returncode == 1 and "no previous snapshot was found" in r.stderr
