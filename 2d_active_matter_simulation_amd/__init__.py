@@ -39,6 +39,16 @@ class AfParams(ctypes.Structure):
     ]
 
 
+class AfSchedule(ctypes.Structure):
+    _fields_ = [("by", ctypes.c_int), ("bx", ctypes.c_int), ("row_radix", ctypes.c_int),
+                ("row_group", ctypes.c_int), ("col_radix", ctypes.c_int), ("col_fwd_group", ctypes.c_int),
+                ("col_inv_group", ctypes.c_int)]
+
+    def as_tuple(self):
+        return (self.by, self.bx, self.row_radix, self.row_group, self.col_radix, self.col_fwd_group,
+                self.col_inv_group)
+
+
 class AfSpec(ctypes.Structure):
     _fields_ = [
         ("T", ctypes.c_double), ("dt", ctypes.c_double), ("Lx", ctypes.c_double), ("Ly", ctypes.c_double),
@@ -83,6 +93,10 @@ def lib() -> ctypes.CDLL:
         L.af_params_reference.restype = None
         L.af_params_jw.argtypes = [ctypes.POINTER(AfParams), ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 8
         L.af_params_jw.restype = None
+        L.af_schedule_count.argtypes = [ctypes.c_int] * 3
+        L.af_schedule_get.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(AfSchedule)]
+        L.af_schedule_select.argtypes = [ctypes.c_int] * 4
+        L.af_schedule_active.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(AfSchedule)]
         L.af_engine_create.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
         L.af_engine_add.argtypes = [vp, ctypes.POINTER(AfParams), dp, dp, ctypes.POINTER(ctypes.c_int)]
         L.af_engine_remove.argtypes = [vp, ctypes.c_int]
@@ -193,6 +207,29 @@ class Run:
         r, c, t = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         _check(lib().af_run_profile(self._h, nsteps, ctypes.byref(r), ctypes.byref(c), ctypes.byref(t)))
         return r.value, c.value, t.value
+
+
+def schedules(precision, nx, ny=None):
+    """The instantiated schedules of a size (index 0 = built-in default)."""
+    ny = nx if ny is None else ny
+    out = []
+    for i in range(lib().af_schedule_count(precision, nx, ny)):
+        sc = AfSchedule()
+        _check(lib().af_schedule_get(precision, nx, ny, i, ctypes.byref(sc)))
+        out.append(sc.as_tuple())
+    return out
+
+
+def select_schedule(precision, nx, ny, index):
+    """Process-wide schedule for runs of this size created afterwards."""
+    _check(lib().af_schedule_select(precision, nx, ny, index))
+
+
+def active_schedule(precision, nx, ny=None):
+    ny = nx if ny is None else ny
+    sc = AfSchedule()
+    _check(lib().af_schedule_active(precision, nx, ny, ctypes.byref(sc)))
+    return sc.as_tuple()
 
 
 def device_bytes(p: AfParams) -> int:

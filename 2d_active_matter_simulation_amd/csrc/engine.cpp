@@ -38,7 +38,7 @@ struct Class {
   size_t desc_bytes = 0;
 };
 
-using ClassKey = std::tuple<int, int, int>;  // precision, nx, ny
+using ClassKey = std::tuple<int, int, int, const Schedule*>;  // precision, nx, ny, schedule
 
 }  // namespace
 
@@ -62,7 +62,7 @@ static void drop_graph(af_engine* e) {
   e->graph_slots.clear();
 }
 
-static ClassKey key_of(const af_params& p) { return ClassKey(p.precision, p.nx, p.ny); }
+static ClassKey key_of(const Slot& s) { return ClassKey(s.p.precision, s.p.nx, s.p.ny, s.g.sched); }
 
 extern "C" {
 
@@ -81,7 +81,7 @@ af_status af_engine_add(af_engine* e, const af_params* p, const double* omega_ha
   af_status st = validate(p);
   if (st != AF_OK) return st;
   Geometry g = geometry(p);
-  if (!g.rc || !g.cc || !g.pc) return fail(AF_EUNSUPPORTED, "no kernel for this size");
+  if (!g.rc || !g.cc) return fail(AF_EUNSUPPORTED, "no kernel for this size");
   if (e->in_use + g.bytes > e->budget) return fail(AF_ENOMEM, "HBM budget exceeded");
   unsigned char* base = nullptr;
   hipError_t he = hipMalloc(&base, g.bytes);
@@ -100,7 +100,7 @@ af_status af_engine_add(af_engine* e, const af_params* p, const double* omega_ha
   s.base = base;
   s.steps = 0;
   e->in_use += g.bytes;
-  Class& c = e->classes[key_of(*p)];
+  Class& c = e->classes[key_of(s)];
   if (!c.stream) {
     he = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
     if (he != hipSuccess) return hip_fail(he, "class stream");
@@ -124,7 +124,7 @@ af_status af_engine_remove(af_engine* e, int slot) {
 static af_status build_descs(af_engine* e, const std::vector<int>& list,
                              std::vector<std::pair<ClassKey, std::vector<int>>>& groups) {
   std::map<ClassKey, std::vector<int>> by;
-  for (int id : list) by[key_of(e->slots[id].p)].push_back(id);
+  for (int id : list) by[key_of(e->slots[id])].push_back(id);
   groups.assign(by.begin(), by.end());
   for (auto& gr : groups) {
     Class& c = e->classes[gr.first];

@@ -5,7 +5,7 @@
 // Element order contract used by every kernel in this library: a lane with
 // index j in [0, L/R) holds, in v[r], the element at position
 //     p = j + r * (L/R),      r = 0..R-1
-// both before and after fft_core().  The first radix pass reads exactly these
+// both before and after fft_multi().  The first radix pass reads exactly these
 // positions (so it can be fed straight from global memory) and the last pass
 // writes exactly these positions (so its output can go straight to global
 // memory or into the elementwise stage maths).  Input and output are in
@@ -90,15 +90,19 @@ struct Dft {
 constexpr int ilog(int n, int b) { return n < b ? 0 : 1 + ilog(n / b, b); }
 constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 
-// Full length-L transform over one LDS area.  `Lds` provides load(p)/store(p,v)
-// for positions p in [0, L) of this lane's transform.  tw points at
-// tw[m] = exp(-2 pi i m / L), m in [0, L).  All lanes of the block must call
-// this together (it contains block barriers).  On entry the LDS area may still
-// be read by other lanes of a previous call only if the caller placed a
-// barrier before; this function's first LDS access is a store after pass 1.
-template <int L, int R, int DIR, class T, class Lds>
-__device__ __forceinline__ void fft_core(cx<T> (&v)[R], int j, Lds& lds, const cx<T>* __restrict__ tw) {
-  static_assert((L & (L - 1)) == 0 && (R & (R - 1)) == 0 && R <= L, "power-of-two sizes");
+// NF length-L transforms done together (one LDS exchange per radix pass for
+// all of them).  `Lds` provides load(f, p, xs) / store(f, p, v, xs) for
+// positions p in [0, L) of field f of this lane's transform, where xs is the
+// stride Ns of the radix pass that stores (so each exchange can use its own
+// bank-conflict-free padding).  tw points at
+// tw[m] = exp(-2 pi i m / L), m in [0, L) (normally an LDS copy).  All lanes
+// of the block must call this together (it contains block barriers).  The
+// first LDS access is a store after radix pass 1, so the caller must place a
+// barrier before the call if other lanes may still be reading the same LDS
+// areas from an earlier use.
+template <int L, int R, int DIR, int NF, class T, class Lds>
+__device__ __forceinline__ void fft_multi(cx<T> (&v)[NF][R], int j, Lds& lds, const cx<T>* tw) {
+  static_assert((L & (L - 1)) == 0 && (R & (R - 1)) == 0 && R <= L && R <= 16, "power-of-two sizes");
   constexpr int NP = ilog(L, R);           // full radix-R passes
   constexpr int RL = L / ipow(R, NP);      // trailing radix (1 if L is a power of R)
   constexpr int S = L / R;                 // lanes per transform
@@ -108,48 +112,75 @@ __device__ __forceinline__ void fft_core(cx<T> (&v)[R], int j, Lds& lds, const c
     if (p > 0) {
       __syncthreads();
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[r] = lds.load(j + r * S);
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[f][r] = lds.load(f, j + r * S, Ns / R);
       const int jm = j & (Ns - 1);
       const int stride = L / (Ns * R);
 #pragma unroll
       for (int r = 1; r < R; ++r) {
         const cx<T> w = tw[r * jm * stride];
-        v[r] = DIR < 0 ? cmul(v[r], w) : cmulc(v[r], w);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) v[f][r] = DIR < 0 ? cmul(v[f][r], w) : cmulc(v[f][r], w);
       }
     }
-    Dft<R, DIR, T>::run(v);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) Dft<R, DIR, T>::run(v[f]);
     const bool last = (p == NP - 1) && (RL == 1);
     if (!last) {
       if (p > 0) __syncthreads();
       const int base = (j / Ns) * Ns * R + (j & (Ns - 1));
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds.store(base + r * Ns, v[r]);
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds.store(f, base + r * Ns, v[f][r], Ns);
     }
     Ns *= R;
   }
   if constexpr (RL > 1) {
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < R; ++m) v[m] = lds.load(j + m * S);
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int m = 0; m < R; ++m) v[f][m] = lds.load(f, j + m * S, Ns / R);
     // R/RL butterflies of radix RL on elements m = q + (R/RL) r'
 #pragma unroll
     for (int q = 0; q < R / RL; ++q) {
       const int b = j + q * S;
-      cx<T> u[RL];
+      cx<T> w[RL];
 #pragma unroll
-      for (int rr = 0; rr < RL; ++rr) {
-        cx<T> x = v[q + (R / RL) * rr];
-        if (rr > 0) {
-          const cx<T> w = tw[rr * b];
-          x = DIR < 0 ? cmul(x, w) : cmulc(x, w);
+      for (int rr = 1; rr < RL; ++rr) w[rr] = tw[rr * b];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        cx<T> u[RL];
+#pragma unroll
+        for (int rr = 0; rr < RL; ++rr) {
+          cx<T> x = v[f][q + (R / RL) * rr];
+          if (rr > 0) x = DIR < 0 ? cmul(x, w[rr]) : cmulc(x, w[rr]);
+          u[rr] = x;
         }
-        u[rr] = x;
-      }
-      Dft<RL, DIR, T>::run(u);
+        Dft<RL, DIR, T>::run(u);
 #pragma unroll
-      for (int rr = 0; rr < RL; ++rr) v[q + (R / RL) * rr] = u[rr];
+        for (int rr = 0; rr < RL; ++rr) v[f][q + (R / RL) * rr] = u[rr];
+      }
     }
   }
 }
+
+// LDS index of element p of column c when W transforms are interleaved
+// (i = p*W + c), padded for the exchange written by a radix-R pass of stride
+// Ns: Ns*W extra elements every Ns*R*W elements, so that the groups of a
+// stride-Ns store land in disjoint bank ranges; no padding once a contiguous
+// run of Ns*W elements spans all 64 four-byte banks.  Conflict-free for the
+// store/load pairs of fft_multi (checked against a 32- and 64-bank model).
+template <int W, int R, int ESZ>
+__device__ __forceinline__ int lds_pad_index(int p, int c, int Ns) {
+  const int i = p * W + c;
+  if (Ns * W * ESZ >= 256) return i;
+  const int g = Ns * R * W;            // power of two
+  return i + (i / g) * (Ns * W);
+}
+// Elements of one LDS area for transforms of length L (covers every padding).
+template <int L, int W, int R> constexpr int lds_area_elems() { return L * W + (L * W) / R + 1; }
 
 }  // namespace af

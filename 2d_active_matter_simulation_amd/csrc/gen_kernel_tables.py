@@ -1,0 +1,141 @@
+#!/usr/bin/env python3
+"""Generates kernels_f32.hip / kernels_f64.hip: the instantiated row/column
+pass kernels and the schedule table of each (precision, Nx/2, Ny).
+
+A schedule fixes the brick shape (BY rows x BX columns) and, for each pass,
+the radix R and how many fields share an LDS exchange.  For every size there
+is one default schedule (used unless a tuned table in schedules.txt says
+otherwise) and, for square grids from 256^2 to 4096^2, a few alternatives for
+the autotuner (autotune.py).  Constraints checked here: at most 512 lanes per
+block and at most 160 KB of LDS per block.
+
+    python3 gen_kernel_tables.py        # rewrites the two .hip files
+"""
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SIZES = [32, 64, 128, 256, 512, 1024, 2048, 4096]
+LDS_MAX = 160 * 1024
+
+
+def lds_area(L, W, R):
+    return L * W + (L * W) // R + 1
+
+
+def lds_row(nx2, rb, R, G, esz):
+    return (G * lds_area(nx2, rb, R) + 2 * nx2) * esz
+
+
+def lds_col(ny, W, R, GF, GI, esz):
+    return (max(GF, GI) * lds_area(ny, W, R) + ny) * esz
+
+
+def default_schedule(prec, nx, ny):
+    """(by, bx, row_r, row_g, col_r, col_gf, col_gi)"""
+    nx2 = nx // 2
+    if prec == 32:
+        by = 2 if nx2 >= 2048 else 4
+        bx = 2
+        row = (4, 3) if nx2 == 16 else ((8, 1) if nx2 >= 2048 else (8, 3))
+        col = (16, 1, 1) if ny >= 4096 else (8, 2, 3)
+    else:
+        by = 2 if nx2 >= 2048 else 4
+        bx = 1 if ny >= 4096 else 2
+        row = (4, 3) if nx2 == 16 else ((8, 1) if nx2 >= 1024 else (8, 3))
+        col = (8, 1, 1) if ny >= 2048 else (8, 2, 3)
+    return (by, bx) + row + col
+
+
+def candidates(prec, n):
+    """Alternative schedules for the autotuner (square n x n only)."""
+    out = [default_schedule(prec, n, n)]
+    nx2 = n // 2
+    if prec == 32:
+        alts = [(4, 2, 4, 3, 4, 2, 3), (4, 4, 8, 3, 8, 2, 3), (2, 4, 8, 3, 8, 2, 3), (4, 2, 8, 1, 8, 1, 1),
+                (4, 2, 8, 3, 16, 1, 1), (2, 2, 8, 3, 8, 2, 3), (4, 2, 4, 3, 8, 2, 3), (4, 2, 8, 3, 4, 2, 3)]
+    else:
+        alts = [(4, 2, 4, 3, 4, 2, 3), (4, 1, 8, 3, 8, 2, 3), (2, 2, 8, 3, 8, 2, 3), (4, 2, 8, 1, 8, 1, 1),
+                (4, 2, 4, 1, 4, 1, 1), (2, 2, 4, 3, 8, 1, 1)]
+    for s in alts:
+        if s not in out and feasible(prec, n, n, s):
+            out.append(s)
+    return [s for s in out if feasible(prec, n, n, s)]
+
+
+def feasible(prec, nx, ny, s):
+    by, bx, rr, rg, cr, gf, gi = s
+    esz = 8 if prec == 32 else 16
+    nx2 = nx // 2
+    if rr > nx2 or cr > ny:
+        return False
+    tr, tc = nx2 // rr, ny // cr
+    if tr % bx or nx2 % bx or ny % by:
+        return False
+    if by * tr > 512 or bx * tc > 512:
+        return False
+    if lds_row(nx2, by, rr, rg, esz) > LDS_MAX or lds_col(ny, bx, cr, gf, gi, esz) > LDS_MAX:
+        return False
+    return True
+
+
+HDR = """// {what} instantiations of the stage kernels and the schedule table.
+// Generated by gen_kernel_tables.py -- edit that script, not this file.
+// Row kernel:    (Nx/2, RB = brick height, R, BX = brick width, G)
+// Column kernel: (Ny, W = brick width, R, BY = brick height, GF, GI)
+// Schedule:      (Nx/2, Ny, BY, BX, row R, row G, col R, col GF, col GI, default)
+#include "kernels_inst.hpp"
+
+namespace af {{
+namespace {{
+using T = {T};
+const RowCfg kRow[] = {{
+{rows}
+}};
+const ColCfg kCol[] = {{
+{cols}
+}};
+const Schedule kSched[] = {{
+{scheds}
+}};
+}}  // namespace
+
+const RowCfg* row_cfgs_{tag}(int* n) {{ *n = (int)(sizeof(kRow) / sizeof(kRow[0])); return kRow; }}
+const ColCfg* col_cfgs_{tag}(int* n) {{ *n = (int)(sizeof(kCol) / sizeof(kCol[0])); return kCol; }}
+const Schedule* schedules_{tag}(int* n) {{ *n = (int)(sizeof(kSched) / sizeof(kSched[0])); return kSched; }}
+ConvLaunch to_bricks_{tag}() {{ return launch_to_bricks<T>; }}
+ConvLaunch from_bricks_{tag}() {{ return launch_from_bricks<T>; }}
+
+}}  // namespace af
+"""
+
+
+def gen(prec):
+    tag = "f32" if prec == 32 else "f64"
+    T = "float" if prec == 32 else "double"
+    scheds = []
+    for nx in SIZES:
+        for ny in SIZES:
+            d = default_schedule(prec, nx, ny)
+            assert feasible(prec, nx, ny, d), (prec, nx, ny, d)
+            scheds.append((nx // 2, ny) + d + (1,))
+            if nx == ny and nx >= 256:
+                for s in candidates(prec, nx)[1:]:
+                    scheds.append((nx // 2, ny) + s + (0,))
+    rows, cols = set(), set()
+    for (nx2, ny, by, bx, rr, rg, cr, gf, gi, _) in scheds:
+        rows.add((nx2, by, rr, bx, rg))
+        cols.add((ny, bx, cr, by, gf, gi))
+    rl = [f"    {{{a}, {b}, {r}, {c}, {b * (a // r)}, launch_row<T, {a}, {b}, {r}, {c}, {g}>, {g}}},"
+          for (a, b, r, c, g) in sorted(rows)]
+    cl = [f"    {{{a}, {w}, {r}, {by}, {w * (a // r)}, launch_col<T, {a}, {w}, {r}, {by}, {gf}, {gi}>, {gf}, {gi}}},"
+          for (a, w, r, by, gf, gi) in sorted(cols)]
+    sl = [f"    {{{a}, {b}, {c}, {d}, {e}, {f}, {g}, {h}, {i}, {j}}}," for (a, b, c, d, e, f, g, h, i, j) in scheds]
+    text = HDR.format(what=f"FP{prec}", T=T, tag=tag, rows="\n".join(rl), cols="\n".join(cl), scheds="\n".join(sl))
+    with open(os.path.join(HERE, f"kernels_{tag}.hip"), "w") as f:
+        f.write(text)
+    return len(rl), len(cl), len(sl)
+
+
+if __name__ == "__main__":
+    for prec in (32, 64):
+        print(prec, gen(prec))

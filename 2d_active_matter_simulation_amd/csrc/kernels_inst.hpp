@@ -5,16 +5,16 @@
 
 namespace af {
 
-template <class T, int NX2, int RB, int R, int BX>
+template <class T, int NX2, int RB, int R, int BX, int G>
 hipError_t launch_row(const void* descs, int ninst, int tiles, const void* pool, hipStream_t s) {
-  hipLaunchKernelGGL((row_pass<T, NX2, RB, R, BX>), dim3(ninst * tiles), dim3(RB * (NX2 / R)), 0, s,
+  hipLaunchKernelGGL((row_pass<T, NX2, RB, R, BX, G>), dim3(ninst * tiles), dim3(RB * (NX2 / R)), 0, s,
                      (const InstDesc<T>*)descs, tiles, (const cx<T>*)pool);
   return hipGetLastError();
 }
 
-template <class T, int NY, int W, int R, int BY>
+template <class T, int NY, int W, int R, int BY, int GF, int GI>
 hipError_t launch_col(const void* descs, int ninst, int tiles, int stage, const void* pool, hipStream_t s) {
-  hipLaunchKernelGGL((col_pass<T, NY, W, R, BY>), dim3(ninst * tiles), dim3(W * (NY / R)), 0, s,
+  hipLaunchKernelGGL((col_pass<T, NY, W, R, BY, GF, GI>), dim3(ninst * tiles), dim3(W * (NY / R)), 0, s,
                      (const InstDesc<T>*)descs, tiles, stage, (const cx<T>*)pool);
   return hipGetLastError();
 }

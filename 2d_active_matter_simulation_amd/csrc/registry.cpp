@@ -1,29 +1,132 @@
-// Precision dispatch over the per-precision launch tables.
+// Schedule lookup over the per-precision launch tables, with the tuned table
+// (schedules.txt next to libactflow.so, written by autotune.py) read once.
 #include "registry.hpp"
+
+#include <dlfcn.h>
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
 
 namespace af {
 
-int brick_by_f32(int nx2);
-int brick_bx_f32(int ny);
-const RowCfg* find_row_cfg_f32(int nx2, int bx);
-const ColCfg* find_col_cfg_f32(int ny, int by);
-const PrecCfg* find_prec_cfg_f32();
-int brick_by_f64(int nx2);
-int brick_bx_f64(int ny);
-const RowCfg* find_row_cfg_f64(int nx2, int bx);
-const ColCfg* find_col_cfg_f64(int ny, int by);
-const PrecCfg* find_prec_cfg_f64();
+const RowCfg* row_cfgs_f32(int* n);
+const ColCfg* col_cfgs_f32(int* n);
+const Schedule* schedules_f32(int* n);
+ConvLaunch to_bricks_f32();
+ConvLaunch from_bricks_f32();
+const RowCfg* row_cfgs_f64(int* n);
+const ColCfg* col_cfgs_f64(int* n);
+const Schedule* schedules_f64(int* n);
+ConvLaunch to_bricks_f64();
+ConvLaunch from_bricks_f64();
 
-int brick_by(int precision, int nx2) { return precision == 64 ? brick_by_f64(nx2) : brick_by_f32(nx2); }
-int brick_bx(int precision, int ny) { return precision == 64 ? brick_bx_f64(ny) : brick_bx_f32(ny); }
-const RowCfg* find_row_cfg(int precision, int nx2, int bx) {
-  return precision == 64 ? find_row_cfg_f64(nx2, bx) : find_row_cfg_f32(nx2, bx);
+namespace {
+
+using Key = std::tuple<int, int, int>;   // precision, nx2, ny
+std::mutex g_mu;
+std::map<Key, const Schedule*> g_selected;
+std::map<Key, const Schedule*> g_tuned;
+bool g_tuned_loaded = false;
+
+const Schedule* table(int precision, int* n) {
+  return precision == 64 ? schedules_f64(n) : schedules_f32(n);
 }
-const ColCfg* find_col_cfg(int precision, int ny, int by) {
-  return precision == 64 ? find_col_cfg_f64(ny, by) : find_col_cfg_f32(ny, by);
+
+std::vector<const Schedule*> list_for(int precision, int nx2, int ny) {
+  int n = 0;
+  const Schedule* s = table(precision, &n);
+  std::vector<const Schedule*> out;
+  for (int i = 0; i < n; ++i)
+    if (s[i].nx2 == nx2 && s[i].ny == ny) {
+      if (s[i].is_default) out.insert(out.begin(), &s[i]); else out.push_back(&s[i]);
+    }
+  return out;
 }
-const PrecCfg* find_prec_cfg(int precision) {
-  return precision == 64 ? find_prec_cfg_f64() : find_prec_cfg_f32();
+
+void load_tuned_locked() {
+  if (g_tuned_loaded) return;
+  g_tuned_loaded = true;
+  Dl_info info;
+  if (!dladdr((void*)&load_tuned_locked, &info) || !info.dli_fname) return;
+  std::string path(info.dli_fname);
+  const size_t slash = path.rfind('/');
+  path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/schedules.txt";
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) return;
+  char line[512];
+  while (std::fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    int prec, nx, ny, v[7];
+    if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d", &prec, &nx, &ny, &v[0], &v[1], &v[2], &v[3], &v[4],
+                    &v[5], &v[6]) != 10)
+      continue;
+    for (const Schedule* s : list_for(prec, nx / 2, ny))
+      if (s->by == v[0] && s->bx == v[1] && s->row_r == v[2] && s->row_g == v[3] && s->col_r == v[4] &&
+          s->col_gf == v[5] && s->col_gi == v[6])
+        g_tuned[Key(prec, nx / 2, ny)] = s;
+  }
+  std::fclose(f);
 }
+
+}  // namespace
+
+int schedule_count(int precision, int nx2, int ny) { return (int)list_for(precision, nx2, ny).size(); }
+
+const Schedule* schedule_at(int precision, int nx2, int ny, int index) {
+  auto l = list_for(precision, nx2, ny);
+  return (index >= 0 && index < (int)l.size()) ? l[index] : nullptr;
+}
+
+const Schedule* active_schedule(int precision, int nx2, int ny) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  const Key k(precision, nx2, ny);
+  auto it = g_selected.find(k);
+  if (it != g_selected.end()) return it->second;
+  load_tuned_locked();
+  auto jt = g_tuned.find(k);
+  if (jt != g_tuned.end()) return jt->second;
+  auto l = list_for(precision, nx2, ny);
+  return l.empty() ? nullptr : l[0];
+}
+
+bool select_schedule(int precision, int nx2, int ny, int index) {
+  if (index < 0) {   // back to the tuned table / default
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_selected.erase(Key(precision, nx2, ny));
+    return true;
+  }
+  const Schedule* s = schedule_at(precision, nx2, ny, index);
+  if (!s) return false;
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_selected[Key(precision, nx2, ny)] = s;
+  return true;
+}
+
+const RowCfg* find_row_cfg(int precision, const Schedule* s) {
+  int n = 0;
+  const RowCfg* c = precision == 64 ? row_cfgs_f64(&n) : row_cfgs_f32(&n);
+  for (int i = 0; i < n; ++i)
+    if (c[i].nx2 == s->nx2 && c[i].rb == s->by && c[i].r == s->row_r && c[i].bx == s->bx && c[i].g == s->row_g)
+      return &c[i];
+  return nullptr;
+}
+
+const ColCfg* find_col_cfg(int precision, const Schedule* s) {
+  int n = 0;
+  const ColCfg* c = precision == 64 ? col_cfgs_f64(&n) : col_cfgs_f32(&n);
+  for (int i = 0; i < n; ++i)
+    if (c[i].ny == s->ny && c[i].w == s->bx && c[i].r == s->col_r && c[i].by == s->by && c[i].gf == s->col_gf &&
+        c[i].gi == s->col_gi)
+      return &c[i];
+  return nullptr;
+}
+
+ConvLaunch to_bricks_launch(int precision) { return precision == 64 ? to_bricks_f64() : to_bricks_f32(); }
+ConvLaunch from_bricks_launch(int precision) { return precision == 64 ? from_bricks_f64() : from_bricks_f32(); }
 
 }  // namespace af

@@ -1,7 +1,7 @@
-// Launch table: one row-pass and one column-pass kernel per (precision, size,
-// brick shape).  A size's brick shape and radix are fixed here, so every run
-// of a given (precision, Nx, Ny) uses the same arithmetic whether it is
-// stepped alone or inside an ensemble launch.
+// Launch table and schedules.  A schedule of a grid size fixes the brick shape
+// (BY x BX), and for each pass the radix and the number of fields per LDS
+// exchange.  Every run of a given (precision, Nx, Ny, schedule) uses the same
+// kernels and the same arithmetic, whether stepped alone or in an ensemble.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstddef>
@@ -16,23 +16,27 @@ using ConvLaunch = hipError_t (*)(const void* src, void* dst, int nx, int ny, in
 struct RowCfg {
   int nx2, rb, r, bx, threads;
   RowLaunch launch;
+  int g;
 };
 struct ColCfg {
   int ny, w, r, by, threads;
   ColLaunch launch;
+  int gf, gi;
 };
-struct PrecCfg {
-  size_t elem_bytes;      // sizeof(complex)
-  ConvLaunch to_bricks;   // reference-layout doubles -> bricks
-  ConvLaunch from_bricks; // bricks -> reference-layout doubles
+struct Schedule {
+  int nx2, ny, by, bx, row_r, row_g, col_r, col_gf, col_gi, is_default;
 };
 
-// Brick shape for a grid: BY follows the row length (rows per row block),
-// BX follows the column length (columns per column block).
-int brick_by(int precision, int nx2);
-int brick_bx(int precision, int ny);
-const RowCfg* find_row_cfg(int precision, int nx2, int bx);
-const ColCfg* find_col_cfg(int precision, int ny, int by);
-const PrecCfg* find_prec_cfg(int precision);
+// Schedules instantiated for (precision, nx2, ny); index 0 is the default.
+int schedule_count(int precision, int nx2, int ny);
+const Schedule* schedule_at(int precision, int nx2, int ny, int index);
+// The schedule new runs of this size use: a process-wide selection, else the
+// tuned table (schedules.txt beside the library), else the default.
+const Schedule* active_schedule(int precision, int nx2, int ny);
+bool select_schedule(int precision, int nx2, int ny, int index);
+const RowCfg* find_row_cfg(int precision, const Schedule* s);
+const ColCfg* find_col_cfg(int precision, const Schedule* s);
+ConvLaunch to_bricks_launch(int precision);
+ConvLaunch from_bricks_launch(int precision);
 
 }  // namespace af

@@ -89,12 +89,16 @@ Geometry geometry(const af_params* p) {
   g.prec = p->precision;
   g.nx2 = p->nx / 2;
   g.ny = p->ny;
-  g.by = brick_by(g.prec, g.nx2);
-  g.bx = brick_bx(g.prec, g.ny);
-  g.rc = find_row_cfg(g.prec, g.nx2, g.bx);
-  g.cc = find_col_cfg(g.prec, g.ny, g.by);
-  g.pc = find_prec_cfg(g.prec);
-  g.esz = g.pc ? g.pc->elem_bytes : 0;
+  g.sched = active_schedule(g.prec, g.nx2, g.ny);
+  if (g.sched) {
+    g.by = g.sched->by;
+    g.bx = g.sched->bx;
+    g.rc = find_row_cfg(g.prec, g.sched);
+    g.cc = find_col_cfg(g.prec, g.sched);
+  }
+  g.to_bricks = to_bricks_launch(g.prec);
+  g.from_bricks = from_bricks_launch(g.prec);
+  g.esz = g.prec == 64 ? 2 * sizeof(double) : 2 * sizeof(float);
   g.nfield = (size_t)g.ny * g.nx2;
   g.row_tiles = g.rc ? g.ny / g.rc->rb : 0;
   g.col_tiles = g.cc ? g.nx2 / g.cc->w : 0;
@@ -184,7 +188,7 @@ af_status load_state(const Geometry& g, unsigned char* base, const double* omega
     if ((e = hipMemcpyAsync(staging, omega_hat, n * sizeof(double), hipMemcpyHostToDevice, stream)) != hipSuccess) {
       st = hip_fail(e, "upload state"); break;
     }
-    if ((e = g.pc->to_bricks(staging, base + g.off_field[0], g.nx2 * 2, g.ny, g.by, g.bx, stream)) != hipSuccess) {
+    if ((e = g.to_bricks(staging, base + g.off_field[0], g.nx2 * 2, g.ny, g.by, g.bx, stream)) != hipSuccess) {
       st = hip_fail(e, "to_bricks"); break;
     }
     if ((e = hipMemcpyAsync(base + g.off_mean, &ms, sizeof(ms), hipMemcpyHostToDevice, stream)) != hipSuccess) {
@@ -207,7 +211,7 @@ af_status store_state(const Geometry& g, unsigned char* base, double* omega_hat,
   af_status st = AF_OK;
   MeanState ms;
   do {
-    if ((e = g.pc->from_bricks(base + g.off_field[0], staging, g.nx2 * 2, g.ny, g.by, g.bx, stream)) != hipSuccess) {
+    if ((e = g.from_bricks(base + g.off_field[0], staging, g.nx2 * 2, g.ny, g.by, g.bx, stream)) != hipSuccess) {
       st = hip_fail(e, "from_bricks"); break;
     }
     if ((e = hipMemcpyAsync(omega_hat, staging, n * sizeof(double), hipMemcpyDeviceToHost, stream)) != hipSuccess) {
@@ -296,7 +300,7 @@ af_status af_run_create(const af_params* p, af_run** out) {
   af_run* r = new af_run();
   r->p = *p;
   r->g = geometry(p);
-  if (!r->g.rc || !r->g.cc || !r->g.pc) { delete r; return fail(AF_EUNSUPPORTED, "no kernel for this size"); }
+  if (!r->g.rc || !r->g.cc) { delete r; return fail(AF_EUNSUPPORTED, "no kernel for this size"); }
   hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete r; return hip_fail(e, "stream create"); }
   e = hipMalloc(&r->base, r->g.bytes);
@@ -423,6 +427,36 @@ af_status af_run_profile(af_run* r, int nsteps, double* row_ms, double* col_ms, 
   }
   for (auto& x : ev) (void)hipEventDestroy(x);
   if (e != hipSuccess) return hip_fail(e, "profile");
+  return AF_OK;
+}
+
+static void fill_schedule(const Schedule* s, af_schedule* out) {
+  out->by = s->by;
+  out->bx = s->bx;
+  out->row_radix = s->row_r;
+  out->row_group = s->row_g;
+  out->col_radix = s->col_r;
+  out->col_fwd_group = s->col_gf;
+  out->col_inv_group = s->col_gi;
+}
+
+int af_schedule_count(int precision, int nx, int ny) { return schedule_count(precision, nx / 2, ny); }
+
+af_status af_schedule_get(int precision, int nx, int ny, int index, af_schedule* out) {
+  const Schedule* s = schedule_at(precision, nx / 2, ny, index);
+  if (!s || !out) return fail(AF_EINVAL, "no such schedule");
+  fill_schedule(s, out);
+  return AF_OK;
+}
+
+af_status af_schedule_select(int precision, int nx, int ny, int index) {
+  return select_schedule(precision, nx / 2, ny, index) ? AF_OK : fail(AF_EINVAL, "no such schedule");
+}
+
+af_status af_schedule_active(int precision, int nx, int ny, af_schedule* out) {
+  const Schedule* s = active_schedule(precision, nx / 2, ny);
+  if (!s || !out) return fail(AF_EINVAL, "no schedule for this size");
+  fill_schedule(s, out);
   return AF_OK;
 }
 

@@ -17,9 +17,10 @@ constexpr int kRingCap = 1024;   // per-run ring of per-step energies
 // single device allocation.
 struct Geometry {
   int prec = 0, nx2 = 0, ny = 0, by = 0, bx = 0, lmax = 0;
+  const Schedule* sched = nullptr;
   const RowCfg* rc = nullptr;
   const ColCfg* cc = nullptr;
-  const PrecCfg* pc = nullptr;
+  ConvLaunch to_bricks = nullptr, from_bricks = nullptr;
   size_t esz = 0, nfield = 0;
   int row_tiles = 0, col_tiles = 0;
   size_t off_field[7] = {0, 0, 0, 0, 0, 0, 0};  // wo, acc, um, vm, wm, pm, qm

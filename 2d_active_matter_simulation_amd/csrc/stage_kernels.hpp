@@ -37,55 +37,61 @@
 
 namespace af {
 
+// Largest |ky index| kept by the mask in column x (-1: none), so the per-mode
+// test of the column pass is one integer compare.  Mode 0: the largest y with
+// 9 (x^2 Ny^2 + y^2 Nx^2) <= Nx^2 Ny^2; mode 1: 3y <= Ny when 3x <= Nx.
+__device__ __forceinline__ int mask_ymax(int x, int nx, int ny, int mode) {
+  const long long X = x, NX = nx, NY = ny;
+  if (mode != 0) return (3 * X <= NX) ? (int)(NY / 3) : -1;
+  const long long rhs = NY * NY * (NX * NX - 9 * X * X);   // >= 9 NX^2 y^2
+  if (rhs < 0) return -1;
+  const long long c = 9 * NX * NX;
+  long long y = (long long)sqrt((double)rhs / (double)c);
+  while (y > 0 && c * y * y > rhs) --y;
+  while (c * (y + 1) * (y + 1) <= rhs) ++y;
+  return (int)y;
+}
+
 __device__ __forceinline__ bool keep_mode(int x, int yabs, int nx, int ny, int mode) {
   const long long X = x, Y = yabs, NX = nx, NY = ny;
   if (mode == 0) return 9 * (X * X * NY * NY + Y * Y * NX * NX) <= NX * NX * NY * NY;
   return 3 * X <= NX && 3 * Y <= NY;
 }
 
-// LDS view for the column pass: W interleaved columns, one pad element every
-// R*W elements to spread the first pass's stride-R stores over the banks.
-template <class T, int W, int R>
-struct LdsCol {
+// LDS view of NF field areas, each holding W interleaved transforms of
+// length L (the W columns of a column block or the RB rows of a row block).
+template <class T, int L, int W, int R>
+struct LdsView {
   cx<T>* base;
   int c;
-  __device__ __forceinline__ int a(int p) const {
-    const int i = p * W + c;
-    return i + i / (R * W);
+  __device__ __forceinline__ int a(int f, int p, int xs) const {
+    return f * lds_area_elems<L, W, R>() + lds_pad_index<W, R, (int)sizeof(cx<T>)>(p, c, xs);
   }
-  __device__ __forceinline__ cx<T> load(int p) const { return base[a(p)]; }
-  __device__ __forceinline__ void store(int p, cx<T> v) { base[a(p)] = v; }
+  __device__ __forceinline__ cx<T> load(int f, int p, int xs) const { return base[a(f, p, xs)]; }
+  __device__ __forceinline__ void store(int f, int p, cx<T> v, int xs) { base[a(f, p, xs)] = v; }
+  // unpadded access for the even/odd pair exchange of the real transforms
+  __device__ __forceinline__ cx<T> load_plain(int f, int p) const { return base[f * lds_area_elems<L, W, R>() + p * W + c]; }
+  __device__ __forceinline__ void store_plain(int f, int p, cx<T> v) { base[f * lds_area_elems<L, W, R>() + p * W + c] = v; }
 };
-template <int NY, int W, int R> constexpr int lds_col_elems() { return NY * W + NY * W / (R * W) + 1; }
 
-// LDS view for the row pass: RB interleaved rows.
-template <class T, int RB, int R>
-struct LdsRow {
-  cx<T>* base;
-  int yy;
-  __device__ __forceinline__ int a(int p) const {
-    const int i = p * RB + yy;
-    return i + i / (R * RB);
-  }
-  __device__ __forceinline__ cx<T> load(int p) const { return base[a(p)]; }
-  __device__ __forceinline__ void store(int p, cx<T> v) { base[a(p)] = v; }
-};
-template <int NX2, int RB, int R> constexpr int lds_row_elems() { return NX2 * RB + NX2 * RB / (R * RB) + 1; }
-
-// Twiddle pool: tw[L + m] = exp(-2 pi i m / L) for every power of two L.
+// Twiddle pool in HBM: tw[L + m] = exp(-2 pi i m / L) for every power of two L.
 template <class T>
 __device__ __forceinline__ const cx<T>* tw_of(const cx<T>* pool, int L) { return pool + L; }
 
-template <class T>
-__device__ __forceinline__ T sq(T a) { return a * a; }
+// Copies n twiddles into LDS (caller places a barrier before their first use).
+template <class T, int NT>
+__device__ __forceinline__ void stage_twiddles(cx<T>* dst, const cx<T>* __restrict__ src, int n) {
+  for (int i = threadIdx.x; i < n; i += NT) dst[i] = src[i];
+}
 
 // Deterministic block sum of NV doubles; result valid in thread 0.
 template <int NV, int NT>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double* red /* LDS, NV*NT/64 */) {
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* red /* LDS, NV * waves */) {
+  constexpr int WW = NT < 64 ? NT : 64;   // lanes of a (possibly partial) wave
 #pragma unroll
   for (int k = 0; k < NV; ++k)
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    for (int off = WW / 2; off >= 1; off >>= 1) v[k] += __shfl_xor(v[k], off, WW);
   constexpr int NW = (NT + 63) / 64;
   if constexpr (NW > 1) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -105,50 +111,51 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red /* LDS, N
 
 // ---------------------------------------------------------------------------
 // Row pass.  Block = RB rows (one brick row) x TR = NX2/R lanes per row.
-// Thread mapping keeps a wave's loads contiguous in the brick layout.
+// G = 3: the three C2R transforms share their LDS exchanges (and the two
+// R2C transforms theirs); G = 1: one field at a time (less LDS).
 // ---------------------------------------------------------------------------
-template <class T, int NX2, int RB, int R, int BX>
+template <class T, int NX2, int RB, int R, int BX, int G>
 __global__ void __launch_bounds__(RB * (NX2 / R))
 row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>* __restrict__ pool) {
   constexpr int TR = NX2 / R;
   constexpr int NT = RB * TR;
-  static_assert(TR % BX == 0, "row lanes must cover whole bricks");
-  __shared__ cx<T> lds_mem[lds_row_elems<NX2, RB, R>()];
+  static_assert(NX2 % BX == 0 && TR % BX == 0, "row lanes must cover whole bricks");
+  static_assert(G == 1 || G == 3, "group of 1 or 3 fields");
+  constexpr int GR = G == 3 ? 2 : 1;   // R2C group
+  __shared__ cx<T> lds_mem[G * lds_area_elems<NX2, RB, R>()];
+  __shared__ cx<T> tw_h[NX2];          // exp(-2 pi i m / (Nx/2)), the half-length transforms
+  __shared__ cx<T> tw_n[NX2];          // exp(-2 pi i k / Nx), k < Nx/2, the even/odd split
   __shared__ double red[4 * ((NT + 63) / 64)];
 
   const int inst = blockIdx.x / tiles_per_inst;
   const int tile = blockIdx.x - inst * tiles_per_inst;
   const InstDesc<T>& d = descs[inst];
+  // lane -> (row yy of the block, transform lane j); a wave still covers a
+  // contiguous range of whole bricks
   const int t = threadIdx.x;
-  const int xx = t % BX;
-  const int yy = (t / BX) % RB;
-  const int jb = t / (BX * RB);
-  const int j = jb * BX + xx;
-  LdsRow<T, RB, R> lds{lds_mem, yy};
-  const cx<T>* twh = tw_of(pool, NX2);       // length-Nx/2 transform
-  const cx<T>* twn = tw_of(pool, 2 * NX2);   // exp(-2 pi i k / Nx)
+  const int yy = t % RB;
+  const int j = t / RB;
+  LdsView<T, NX2, RB, R> lds{lds_mem, yy};
 
   // brick address of (row tile, position k) for this lane's row
-  const size_t tile_base = (size_t)tile * (NX2 / BX) * (RB * BX) + yy * BX + xx;
-  auto addr = [&](int k) -> size_t { return tile_base + (size_t)(k / BX) * (RB * BX); };
+  const size_t tile_base = (size_t)tile * (NX2 / BX) * (RB * BX) + yy * BX;
+  auto addr = [&](int k) -> size_t { return tile_base + (size_t)(k / BX) * (RB * BX) + (k % BX); };
 
-  cx<T> u[R], v[R], w[R];
+  cx<T> f3[3][R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const size_t a = addr(j + r * TR);
-    u[r] = d.um[a];
-    v[r] = d.vm[a];
-    w[r] = d.wm[a];
+    f3[0][r] = d.um[a];
+    f3[1][r] = d.vm[a];
+    f3[2][r] = d.wm[a];
   }
+  stage_twiddles<T, NT>(tw_h, tw_of(pool, NX2), NX2);
+  stage_twiddles<T, NT>(tw_n, tw_of(pool, 2 * NX2), NX2);
 
   // Row C2R of a half spectrum X[0..NX2) (X[NX2] = 0, Im X[0] ignored):
   // Z[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/N} (X[k] - conj X[M-k]),
   // then inverse length-M transform -> (x[2n], x[2n+1]).
-  auto c2r = [&](cx<T>(&q)[R]) {
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < R; ++r) lds.store(j + r * TR, q[r]);
-    __syncthreads();
+  auto c2r_pre = [&](cx<T>(&q)[R], int f) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int k = j + r * TR;
@@ -157,69 +164,105 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
         xk.im = (T)0;
         xr = cx<T>{(T)0, (T)0};
       } else {
-        xr = cconj(lds.load(NX2 - k));
+        xr = cconj(lds.load_plain(f, NX2 - k));
       }
       const cx<T> s = xk + xr, dd = xk - xr;
-      q[r] = s + cmuli(cmulc(dd, twn[k]));
+      q[r] = s + cmuli(cmulc(dd, tw_n[k]));
     }
-    __syncthreads();
-    fft_core<NX2, R, +1>(q, j, lds, twh);
   };
-  c2r(u);
-  c2r(v);
-  c2r(w);
+  if constexpr (G == 3) {
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds.store_plain(f, j + r * TR, f3[f][r]);
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < 3; ++f) c2r_pre(f3[f], f);
+    __syncthreads();
+    fft_multi<NX2, R, +1, 3>(f3, j, lds, tw_h);
+  } else {
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      cx<T>(&q)[1][R] = *reinterpret_cast<cx<T>(*)[1][R]>(&f3[f]);
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds.store_plain(0, j + r * TR, q[0][r]);
+      __syncthreads();
+      c2r_pre(q[0], 0);
+      __syncthreads();
+      fft_multi<NX2, R, +1, 1>(q, j, lds, tw_h);
+    }
+  }
 
   // Real-space products (nonl_subfunc_1) and the two combined fields.
   const T lam = (T)d.lam, bet = (T)d.beta;
   double esum = 0, gxs = 0, gys = 0;
+  cx<T> f2[2][R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    T f1[2], f2[2];
-    const T uxa[2] = {u[r].re, u[r].im}, uya[2] = {v[r].re, v[r].im}, wa[2] = {w[r].re, w[r].im};
+    T o1[2], o2[2];
+    const T uxa[2] = {f3[0][r].re, f3[0][r].im}, uya[2] = {f3[1][r].re, f3[1][r].im};
+    const T wa[2] = {f3[2][r].re, f3[2][r].im};
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const T ux = uxa[h], uy = uya[h], wv = wa[h];
       const T A = wv * ux, Wp = wv * uy;
       const T Q = ux * ux + uy * uy;
       const T Cx = Q * ux, Cy = Q * uy;
-      f1[h] = lam * A + bet * Cy;
-      f2[h] = lam * Wp - bet * Cx;
+      o1[h] = lam * A + bet * Cy;
+      o2[h] = lam * Wp - bet * Cx;
       esum += (double)ux * (double)ux + (double)uy * (double)uy;
       gxs += (double)Cx;
       gys += (double)Cy;
     }
-    u[r] = cx<T>{f1[0], f1[1]};
-    v[r] = cx<T>{f2[0], f2[1]};
+    f2[0][r] = cx<T>{o1[0], o1[1]};
+    f2[1][r] = cx<T>{o2[0], o2[1]};
   }
 
   // Row R2C of packed real pairs: Z = FFT_M(z), then
   // X[k] = 1/2 (Z[k] + conj Z[M-k]) - 1/2 i e^{-2 pi i k/N} (Z[k] - conj Z[M-k]).
-  auto r2c = [&](cx<T>(&q)[R]) {
-    __syncthreads();
-    fft_core<NX2, R, -1>(q, j, lds, twh);
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < R; ++r) lds.store(j + r * TR, q[r]);
-    __syncthreads();
+  auto r2c_post = [&](cx<T>(&q)[R], int f) {
     const T h = (T)0.5;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int k = j + r * TR;
-      const cx<T> zk = q[r], zm = cconj(lds.load((NX2 - k) & (NX2 - 1)));
+      const cx<T> zk = q[r], zm = cconj(lds.load_plain(f, (NX2 - k) & (NX2 - 1)));
       const cx<T> a = cscale(h, zk + zm), dd = zk - zm;
-      q[r] = a + cmul(cmuli(cscale(-h, dd)), twn[k]);
+      q[r] = a + cmul(cmuli(cscale(-h, dd)), tw_n[k]);
     }
   };
-  r2c(u);
-  r2c(v);
+  if constexpr (GR == 2) {
+    __syncthreads();
+    fft_multi<NX2, R, -1, 2>(f2, j, lds, tw_h);
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds.store_plain(f, j + r * TR, f2[f][r]);
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < 2; ++f) r2c_post(f2[f], f);
+  } else {
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      cx<T>(&q)[1][R] = *reinterpret_cast<cx<T>(*)[1][R]>(&f2[f]);
+      __syncthreads();
+      fft_multi<NX2, R, -1, 1>(q, j, lds, tw_h);
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds.store_plain(0, j + r * TR, q[0][r]);
+      __syncthreads();
+      r2c_post(q[0], 0);
+    }
+  }
 
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int k = j + r * TR;
     const T kx = (T)(d.tkx * (double)k);
     const size_t a = addr(k);
-    d.pm[a] = cx<T>{kx * u[r].im, -kx * u[r].re};   // -i kx X1
-    d.qm[a] = v[r];
+    d.pm[a] = cx<T>{kx * f2[0][r].im, -kx * f2[0][r].re};   // -i kx X1
+    d.qm[a] = f2[1][r];
   }
 
   double sums[3] = {esum, gxs, gys};
@@ -235,14 +278,18 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 // ---------------------------------------------------------------------------
 // Column pass.  Block = W adjacent columns x TC = NY/R lanes per column.
 // stage 0: initial pass only (v^ from the state, inverse transforms).
-// stage 1..4: full stage.
+// stage 1..4: full stage.  GF = 2 / GI = 3: the forward (P, Q) and inverse
+// (v_x, v_y, omega) transforms share LDS exchanges; 1: one at a time.
 // ---------------------------------------------------------------------------
-template <class T, int NY, int W, int R, int BY>
+template <class T, int NY, int W, int R, int BY, int GF, int GI>
 __global__ void __launch_bounds__(W * (NY / R))
 col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, const cx<T>* __restrict__ pool) {
   constexpr int TC = NY / R;
   constexpr int NT = W * TC;
-  __shared__ cx<T> lds_mem[lds_col_elems<NY, W, R>()];
+  constexpr int GMAX = GF > GI ? GF : GI;
+  static_assert((GF == 1 || GF == 2) && (GI == 1 || GI == 3), "field groups");
+  __shared__ cx<T> lds_mem[GMAX * lds_area_elems<NY, W, R>()];
+  __shared__ cx<T> tw_y[NY];
 
   const int inst = blockIdx.x / tiles_per_inst;
   const int tile = blockIdx.x - inst * tiles_per_inst;
@@ -251,8 +298,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
   const int c = t % W, j = t / W;
   const int x = tile * W + c;
   const int nx2 = d.nx >> 1;
-  LdsCol<T, W, R> lds{lds_mem, c};
-  const cx<T>* twy = tw_of(pool, NY);
+  LdsView<T, NY, W, R> lds{lds_mem, c};
 
   const size_t col_base = (size_t)tile * (BY * W) + c;
   const size_t brow = (size_t)(nx2 / W) * (BY * W);  // elements per brick row
@@ -260,39 +306,41 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
 
   const double dt = d.dt;
   const bool col0 = (x == 0);
+  constexpr int RW = NT < 64 ? NT : 64;           // lanes of the first (possibly partial) wave
+  const bool mean_wave = (tile == 0 && t < RW);   // owns the k = 0 mode and the mean ODE
+  const double kxd = d.tkx * (double)x;
+  const double kx2 = kxd * kxd;
+  const int ymax = mask_ymax(x, d.nx, d.ny, d.mask_mode);
   cx<T> sv[R];   // spectral input of the next stage
   double minj[2] = {0.0, 0.0};
 
   if (stage == 0) {
 #pragma unroll
     for (int r = 0; r < R; ++r) sv[r] = d.wo[addr(j + r * TC)];
+    stage_twiddles<T, NT>(tw_y, tw_of(pool, NY), NY);
     if (tile == 0 && t == 0) {
       minj[0] = d.mean->M[0];
       minj[1] = d.mean->M[1];
     }
+    __syncthreads();
   } else {
-    cx<T> pv[R], qv[R];
+    cx<T> pq[2][R], wv[R], av[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const size_t a = addr(j + r * TC);
-      pv[r] = d.pm[a];
-      qv[r] = d.qm[a];
+      pq[0][r] = d.pm[a];
+      pq[1][r] = d.qm[a];
     }
-    fft_core<NY, R, -1>(pv, j, lds, twy);
-    __syncthreads();
-    fft_core<NY, R, -1>(qv, j, lds, twy);
-    // N^ = (P^ - i ky Q^) * mask; pv then holds N^.
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int y = j + r * TC;
-      const int nys = (y <= NY / 2) ? y : y - NY;
-      const double kyd = d.tky * (double)nys;
-      const T keepf = keep_mode(x, nys < 0 ? -nys : nys, d.nx, d.ny, d.mask_mode) ? (T)1 : (T)0;
-      pv[r] = cscale(keepf, pv[r] + cmuli(cscale((T)(-kyd), qv[r])));
+    // row-block partials for the mean ODE, summed per lane in a fixed order
+    double s0 = 0, s1 = 0, s2 = 0;
+    if (mean_wave) {
+      for (int i = t; i < d.row_tiles; i += RW) {
+        s0 += d.partials[(size_t)i * 4 + 0];
+        s1 += d.partials[(size_t)i * 4 + 1];
+        s2 += d.partials[(size_t)i * 4 + 2];
+      }
     }
-    // State: omega_old (stages 1-3), acc (stages 2-4), and the projection
-    // partner (Ny - y) of the kx = 0 column.
-    cx<T> wv[R], av[R], pw[R];
+    cx<T> pw[R];   // projection partner (Ny - y) of the kx = 0 column
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const size_t a = addr(j + r * TC);
@@ -303,18 +351,44 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
         pw[r] = (stage == 1) ? d.wo[addr(yp)] : d.acc[addr(yp)];
       }
     }
-
+    double Mcur[2] = {0, 0}, Mold[2] = {0, 0};
+    long long nstep = 0;
+    if (tile == 0 && t == 0) {
+      Mcur[0] = d.mean->M[0];
+      Mcur[1] = d.mean->M[1];
+      Mold[0] = d.mean->Mo[0];
+      Mold[1] = d.mean->Mo[1];
+      nstep = d.mean->nstep;
+    }
+    stage_twiddles<T, NT>(tw_y, tw_of(pool, NY), NY);
+    __syncthreads();
+    if constexpr (GF == 2) {
+      fft_multi<NY, R, -1, 2>(pq, j, lds, tw_y);
+    } else {
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        cx<T>(&q)[1][R] = *reinterpret_cast<cx<T>(*)[1][R]>(&pq[f]);
+        if (f) __syncthreads();
+        fft_multi<NY, R, -1, 1>(q, j, lds, tw_y);
+      }
+    }
     cx<T> an[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int y = j + r * TC;
       const int nys = (y <= NY / 2) ? y : y - NY;
-      const double kxd = d.tkx * (double)x, kyd = d.tky * (double)nys;
-      const double k2 = kxd * kxd + kyd * kyd;
-      const cx<T> nh = pv[r];
-      const double sig = d.c0 + d.c2 * k2 + d.c4 * (k2 * k2);
-      const T E = (T)exp(sig * dt);
-      const T E2 = (T)exp(sig * dt * 0.5);
+      const double kyd = d.tky * (double)nys;
+      const double k2 = kx2 + kyd * kyd;
+      const T keepf = ((nys < 0 ? -nys : nys) <= ymax) ? (T)1 : (T)0;
+      // N^ = (P^ - i ky Q^) * mask
+      const cx<T> nh = cscale(keepf, pq[0][r] + cmuli(cscale((T)(-kyd), pq[1][r])));
+      T E = (T)1, E2 = (T)1;
+      if (stage <= 3) {
+        const double sig = d.c0 + d.c2 * k2 + d.c4 * (k2 * k2);
+        const double e2d = exp(sig * dt * 0.5);
+        E2 = (T)e2d;
+        E = (T)(e2d * e2d);
+      }
       const bool proj = col0 && (y & (NY / 2 - 1)) != 0;   // y not 0 or Ny/2
       cx<T> a;
       if (stage == 1) {
@@ -347,26 +421,19 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
     for (int r = 0; r < R; ++r) dst[addr(j + r * TC)] = an[r];
 
     // Mean-velocity ODE (host updates of .cu:729-821), FP64, tile 0 only.
-    if (tile == 0 && t < 64) {
-      const int nt = d.row_tiles;
-      double s0 = 0, s1 = 0, s2 = 0;
-      for (int i = t; i < nt; i += 64) {
-        s0 += d.partials[(size_t)i * 4 + 0];
-        s1 += d.partials[(size_t)i * 4 + 1];
-        s2 += d.partials[(size_t)i * 4 + 2];
-      }
+    if (mean_wave) {
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        s0 += __shfl_xor(s0, off, 64);
-        s1 += __shfl_xor(s1, off, 64);
-        s2 += __shfl_xor(s2, off, 64);
+      for (int off = RW / 2; off >= 1; off >>= 1) {
+        s0 += __shfl_xor(s0, off, RW);
+        s1 += __shfl_xor(s1, off, RW);
+        s2 += __shfl_xor(s2, off, RW);
       }
       if (t == 0) {
         MeanState* ms = d.mean;
         const double g[2] = {-d.beta * s1, -d.beta * s2};
-        double M[2] = {ms->M[0], ms->M[1]}, Mo[2] = {ms->Mo[0], ms->Mo[1]}, mm[2];
+        double M[2] = {Mcur[0], Mcur[1]}, Mo[2] = {Mold[0], Mold[1]}, mm[2];
         if (stage == 1) {
-          d.energy[ms->nstep % d.ring_cap] = s0 * d.invN * 0.5;
+          d.energy[nstep % d.ring_cap] = s0 * d.invN * 0.5;
           Mo[0] = M[0];
           Mo[1] = M[1];
           if (d.mean_scheme == 1) { M[0] = d.e * Mo[0]; M[1] = d.e * Mo[1]; }
@@ -383,8 +450,8 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
         ms->m[0] = mm[0]; ms->m[1] = mm[1];
         if (stage == 4) {
           // check_func (.cu:884-897) on the new Re w^[0], owned by this lane
-          if (ms->nan_step < 0 && an[0].re != an[0].re) ms->nan_step = ms->nstep + 1;
-          ms->nstep += 1;
+          if (ms->nan_step < 0 && an[0].re != an[0].re) ms->nan_step = nstep + 1;
+          ms->nstep = nstep + 1;
         }
         minj[0] = mm[0];
         minj[1] = mm[1];
@@ -392,31 +459,57 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
     }
   }
 
-  // Biot-Savart + mean injection + three column inverse transforms.
-  const T invN = (T)d.invN;
+  // Biot-Savart + mean injection + three column inverse transforms:
+  // v_x^ = i ky s^/k^2, v_y^ = -i kx s^/k^2 (nonl_subfunc_0, .cu:1176-1180),
+  // scaled by 1/N so the row pass sees physical fields.
+  T fx[R], fy[R];
 #pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    cx<T> q[R];
+  for (int r = 0; r < R; ++r) {
+    const int y = j + r * TC;
+    const int nys = (y <= NY / 2) ? y : y - NY;
+    const double kyd = d.tky * (double)nys;
+    const double k2 = kx2 + kyd * kyd;
+    const double ik = (k2 > 0.0) ? d.invN / k2 : 0.0;
+    fx[r] = (T)(kyd * ik);
+    fy[r] = (T)(-kxd * ik);
+  }
+  const T invN = (T)d.invN;
+  auto make = [&](cx<T>(&q)[R], int f) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const int y = j + r * TC;
-      const int nys = (y <= NY / 2) ? y : y - NY;
-      const double kxd = d.tkx * (double)x, kyd = d.tky * (double)nys;
-      const double k2 = kxd * kxd + kyd * kyd;
       if (f == 2) {
         q[r] = cscale(invN, sv[r]);
-      } else if (col0 && y == 0) {
+      } else if (col0 && j + r * TC == 0) {
         q[r] = cx<T>{(T)(minj[f] * d.invN), (T)0};
       } else {
-        const double fac = (f == 0 ? kyd : -kxd) / k2 * d.invN;
-        q[r] = cscale((T)fac, cmuli(sv[r]));
+        q[r] = cscale(f == 0 ? fx[r] : fy[r], cmuli(sv[r]));
       }
     }
-    __syncthreads();
-    fft_core<NY, R, +1>(q, j, lds, twy);
-    cx<T>* dst = (f == 0) ? d.um : (f == 1) ? d.vm : d.wm;
+  };
+  if constexpr (GI == 3) {
+    cx<T> q3[3][R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) dst[addr(j + r * TC)] = q[r];
+    for (int f = 0; f < 3; ++f) make(q3[f], f);
+    __syncthreads();
+    fft_multi<NY, R, +1, 3>(q3, j, lds, tw_y);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const size_t a = addr(j + r * TC);
+      d.um[a] = q3[0][r];
+      d.vm[a] = q3[1][r];
+      d.wm[a] = q3[2][r];
+    }
+  } else {
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      cx<T> q[1][R];
+      make(q[0], f);
+      __syncthreads();
+      fft_multi<NY, R, +1, 1>(q, j, lds, tw_y);
+      cx<T>* dst = (f == 0) ? d.um : (f == 1) ? d.vm : d.wm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[addr(j + r * TC)] = q[0][r];
+    }
   }
 }
 

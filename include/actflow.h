@@ -98,6 +98,23 @@ af_status af_run_use_graph(af_run* r, int on);
 /* Bytes of device memory held by a run with these parameters. */
 size_t af_run_device_bytes(const af_params* p);
 
+/* ---- schedules (no reference counterpart: cuFFT plans, .cu:504-508) ----
+ * A schedule fixes the brick shape of the fields in HBM and, per pass, the
+ * radix and how many fields share an LDS exchange.  Index 0 is the built-in
+ * default; runs take the schedule active for their size when created
+ * (af_schedule_select, else the tuned table schedules.txt written by
+ * autotune.py next to the library, else the default). */
+typedef struct {
+  int by, bx;               /* brick rows x columns                           */
+  int row_radix, row_group; /* row pass radix, fields per LDS exchange        */
+  int col_radix, col_fwd_group, col_inv_group;
+} af_schedule;
+int af_schedule_count(int precision, int nx, int ny);
+af_status af_schedule_get(int precision, int nx, int ny, int index, af_schedule* out);
+/* Process-wide choice for runs created afterwards; index -1 clears it. */
+af_status af_schedule_select(int precision, int nx, int ny, int index);
+af_status af_schedule_active(int precision, int nx, int ny, af_schedule* out);
+
 /* ---- ensemble engine: many resident runs, batched launches ------------- */
 typedef struct af_engine af_engine;
 /* hbm_budget: bytes the engine may hold in run state (admission limit). */

@@ -1,0 +1,64 @@
+"""Every instantiated schedule (brick shape, radix, LDS grouping) of a size
+computes the same step: FP64 against the oracle to 1e-12 after 20 steps,
+FP32 against the FP64 oracle to 2e-6."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+JW = dict(lam=3.5, alpha=-0.8, beta=0.01, gamma0=-0.045, gamma2=2.5e-5)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("n", [256, 512])
+def test_every_schedule_matches_oracle(af, orc, prec, n):
+    p = af.params(n, dt=2e-4, precision=prec, **JW)
+    w0 = af.ic_bandlimited(n, seed=9, amp=2.0, kmax=12.0)
+    M0 = (0.2 * n * n, -0.1 * n * n)
+    q = af.AfParams.from_buffer_copy(p)
+    q.precision = 64
+    o = orc.Oracle.from_params(q, nthreads=4)
+    o.set_state(w0, M0)
+    eo = o.step(20, energy=True)
+    wo, mo = o.get_state()
+    tol = 1e-12 if prec == 64 else 2e-6
+    try:
+        for i, sc in enumerate(af.schedules(prec, n)):
+            af.select_schedule(prec, n, n, i)
+            run = af.Run(p)
+            run.set_state(w0, M0)
+            e = run.step(20, energy=True)
+            w, m = run.get_state()
+            run.close()
+            assert rel_l2(w, wo) <= tol, (i, sc, rel_l2(w, wo))
+            assert np.abs(e - eo).max() <= tol * np.abs(eo).max(), (i, sc)
+            assert np.allclose(m, mo, rtol=10 * tol), (i, sc)
+    finally:
+        af.select_schedule(prec, n, n, -1)
+
+
+def test_narrow_blocks_reduce_every_partial(af, orc):
+    """A column block narrower than a wave (Ny = 256, W = 1, radix 8: 32
+    lanes) must still fold every row-block partial of the mean-velocity RHS."""
+    n = 256
+    idx = [i for i, sc in enumerate(af.schedules(64, n)) if sc[1] == 1]
+    if not idx:
+        pytest.skip("no brick-width-1 schedule at 256^2")
+    p = af.params(n, dt=2e-4, precision=64, **JW)
+    w0 = af.ic_bandlimited(n, seed=2, amp=2.0, kmax=12.0)
+    M0 = (0.3 * n * n, 0.2 * n * n)
+    try:
+        af.select_schedule(64, n, n, idx[0])
+        run = af.Run(p)
+        run.set_state(w0, M0)
+        run.step(10)
+        w, m = run.get_state()
+    finally:
+        af.select_schedule(64, n, n, -1)
+    o = orc.Oracle.from_params(p, nthreads=4)
+    o.set_state(w0, M0)
+    o.step(10)
+    wo, mo = o.get_state()
+    assert np.allclose(m, mo, rtol=1e-12)
+    assert rel_l2(w, wo) <= 1e-12
