@@ -49,6 +49,31 @@ class AfSchedule(ctypes.Structure):
                 self.col_inv_group)
 
 
+class AfRequest(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("params", AfParams), ("horizon", ctypes.c_int),
+                ("snapshot_every", ctypes.c_int), ("seed", ctypes.c_ulonglong), ("ic_amp", ctypes.c_double),
+                ("ic_kmax", ctypes.c_double)]
+
+
+class AfRequestResult(ctypes.Structure):
+    _fields_ = [("admit_s", ctypes.c_double), ("finish_s", ctypes.c_double), ("step_p50_s", ctypes.c_double),
+                ("step_p99_s", ctypes.c_double), ("snapshots", ctypes.c_int), ("status", ctypes.c_int),
+                ("M", ctypes.c_double * 2)]
+
+
+class AfServeStats(ctypes.Structure):
+    _fields_ = [("wall_s", ctypes.c_double), ("grid_point_steps", ctypes.c_double), ("throughput", ctypes.c_double),
+                ("step_p50_s", ctypes.c_double), ("step_p99_s", ctypes.c_double), ("ttc_p50_s", ctypes.c_double),
+                ("ttc_p99_s", ctypes.c_double), ("queue_p50_s", ctypes.c_double), ("queue_p99_s", ctypes.c_double),
+                ("max_resident", ctypes.c_int), ("ticks", ctypes.c_int), ("completed", ctypes.c_int),
+                ("peak_bytes", ctypes.c_size_t)]
+
+
+SNAPSHOT_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong,
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                               ctypes.c_int)
+
+
 class AfSpec(ctypes.Structure):
     _fields_ = [
         ("T", ctypes.c_double), ("dt", ctypes.c_double), ("Lx", ctypes.c_double), ("Ly", ctypes.c_double),
@@ -86,6 +111,8 @@ def lib() -> ctypes.CDLL:
         L.af_run_destroy.argtypes = [vp]
         L.af_run_destroy.restype = None
         L.af_run_profile.argtypes = [vp, ctypes.c_int, dp, dp, dp]
+        L.af_run_diagnostics.argtypes = [vp, dp, dp, dp, ctypes.c_int]
+        L.af_engine_diagnostics.argtypes = [vp, ctypes.c_int, dp, dp, dp, ctypes.c_int]
         L.af_run_use_graph.argtypes = [vp, ctypes.c_int]
         L.af_run_device_bytes.argtypes = [ctypes.POINTER(AfParams)]
         L.af_run_device_bytes.restype = ctypes.c_size_t
@@ -97,6 +124,8 @@ def lib() -> ctypes.CDLL:
         L.af_schedule_get.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(AfSchedule)]
         L.af_schedule_select.argtypes = [ctypes.c_int] * 4
         L.af_schedule_active.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(AfSchedule)]
+        L.af_serve.argtypes = [ctypes.POINTER(AfRequest), ctypes.c_int, ctypes.c_size_t, ctypes.c_int, SNAPSHOT_FN,
+                               vp, ctypes.POINTER(dp), ctypes.POINTER(AfRequestResult), ctypes.POINTER(AfServeStats)]
         L.af_engine_create.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
         L.af_engine_add.argtypes = [vp, ctypes.POINTER(AfParams), dp, dp, ctypes.POINTER(ctypes.c_int)]
         L.af_engine_remove.argtypes = [vp, ctypes.c_int]
@@ -198,6 +227,14 @@ class Run:
     def steps_done(self) -> int:
         return lib().af_run_steps_done(self._h)
 
+    def diagnostics(self, nshell: int = 0):
+        """(energy 1/2<|u|^2>, enstrophy 1/2<w^2>, shell spectrum E(k) or None)."""
+        e, z = ctypes.c_double(), ctypes.c_double()
+        spec = np.zeros(max(nshell, 1))
+        _check(lib().af_run_diagnostics(self._h, ctypes.byref(e), ctypes.byref(z),
+                                        _dp(spec) if nshell else None, nshell))
+        return e.value, z.value, (spec if nshell else None)
+
     def use_graph(self, on: bool):
         _check(lib().af_run_use_graph(self._h, 1 if on else 0))
 
@@ -289,6 +326,13 @@ class Engine:
     def steps_done(self, slot: int) -> int:
         return lib().af_engine_steps_done(self._h, slot)
 
+    def diagnostics(self, slot: int, nshell: int = 0):
+        e, z = ctypes.c_double(), ctypes.c_double()
+        spec = np.zeros(max(nshell, 1))
+        _check(lib().af_engine_diagnostics(self._h, slot, ctypes.byref(e), ctypes.byref(z),
+                                           _dp(spec) if nshell else None, nshell))
+        return e.value, z.value, (spec if nshell else None)
+
     @property
     def bytes_in_use(self) -> int:
         return lib().af_engine_bytes_in_use(self._h)
@@ -296,6 +340,55 @@ class Engine:
     @property
     def count(self) -> int:
         return lib().af_engine_count(self._h)
+
+
+# ---- serving ---------------------------------------------------------------
+
+@dataclass
+class Request:
+    arrival_s: float
+    params: AfParams
+    horizon: int
+    snapshot_every: int = 0
+    seed: int = 1
+    ic_amp: float = 1e-3
+    ic_kmax: float = 10.0
+
+
+def serve(requests, hbm_budget, max_tick_steps=50, on_snapshot=None, keep_final=False):
+    """Run requests through the continuous-batching loop (af_serve).
+    on_snapshot(request_index, step, omega_hat, M) receives every snapshot.
+    Returns (per-request result dicts, stats dict, final states or None)."""
+    n = len(requests)
+    arr = (AfRequest * max(n, 1))()
+    for i, r in enumerate(requests):
+        arr[i].arrival_s = r.arrival_s
+        arr[i].params = r.params
+        arr[i].horizon = r.horizon
+        arr[i].snapshot_every = r.snapshot_every
+        arr[i].seed = r.seed
+        arr[i].ic_amp = r.ic_amp
+        arr[i].ic_kmax = r.ic_kmax
+    res = (AfRequestResult * max(n, 1))()
+    stats = AfServeStats()
+    finals = None
+    fptr = None
+    if keep_final:
+        finals = [np.zeros((r.params.ny, r.params.nx // 2 + 1), dtype=np.complex128) for r in requests]
+        fptr = (ctypes.POINTER(ctypes.c_double) * max(n, 1))(*[_dp(f.view(np.float64)) for f in finals])
+
+    def _cb(user, req, step, w, M, nx, ny):
+        if on_snapshot is not None:
+            a = np.ctypeslib.as_array(w, shape=(ny * (nx // 2 + 1) * 2,)).copy()
+            on_snapshot(req, step, a.view(np.complex128).reshape(ny, nx // 2 + 1), (M[0], M[1]))
+
+    cb = SNAPSHOT_FN(_cb)
+    _check(lib().af_serve(arr, n, hbm_budget, max_tick_steps, cb, None, fptr, res, ctypes.byref(stats)))
+    out = [dict(admit_s=res[i].admit_s, finish_s=res[i].finish_s, step_p50_s=res[i].step_p50_s,
+                step_p99_s=res[i].step_p99_s, snapshots=res[i].snapshots, status=res[i].status,
+                M=(res[i].M[0], res[i].M[1])) for i in range(n)]
+    st = {k: getattr(stats, k) for k, _ in AfServeStats._fields_}
+    return out, st, finals
 
 
 # ---- host formats --------------------------------------------------------

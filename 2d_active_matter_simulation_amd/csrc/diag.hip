@@ -1,0 +1,105 @@
+// Diagnostics of a run's current state (no reference counterpart; the
+// reference only writes the energy, .cu:1118-1127): energy 1/2<|u|^2>,
+// enstrophy 1/2<omega^2> and the shell spectrum E(k), unit-width shells in
+// |k|/k_unit.  Energy and enstrophy are deterministic FP64 block sums reduced
+// in block order on the host; spectrum bins are FP64 atomic sums (order of
+// additions not fixed, ~1e-16 relative).
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "run_impl.hpp"
+
+namespace af {
+
+template <class T>
+__global__ void __launch_bounds__(256) diag_kernel(const cx<T>* __restrict__ w, int nx2, int ny, int by, int bx,
+                                                   double tkx, double tky, double kunit, double invN2,
+                                                   double* __restrict__ part, double* __restrict__ spec, int nshell) {
+  double e = 0, z = 0;
+  const size_t n = (size_t)ny * nx2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int y = (int)(i / nx2), x = (int)(i % nx2);
+    const size_t a = ((size_t)(y / by) * (nx2 / bx) + x / bx) * (size_t)(by * bx) + (y % by) * bx + x % bx;
+    const double re = (double)w[a].re, im = (double)w[a].im;
+    const int nys = (y <= ny / 2) ? y : y - ny;
+    const double kx = tkx * x, ky = tky * nys, k2 = kx * kx + ky * ky;
+    const double wt = (x == 0) ? 1.0 : 2.0;
+    const double a2 = wt * (re * re + im * im) * invN2;
+    z += 0.5 * a2;
+    if (k2 > 0) {
+      const double ek = 0.5 * a2 / k2;
+      e += ek;
+      const int s = (int)floor(sqrt(k2) / kunit + 0.5);
+      if (s < nshell) atomicAdd(&spec[s], ek);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    e += __shfl_xor(e, off, 64);
+    z += __shfl_xor(z, off, 64);
+  }
+  __shared__ double red[2 * 4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) { red[2 * wv] = e; red[2 * wv + 1] = z; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double se = red[0], sz = red[1];
+    for (int k = 1; k < 4; ++k) { se += red[2 * k]; sz += red[2 * k + 1]; }
+    part[2 * blockIdx.x] = se;
+    part[2 * blockIdx.x + 1] = sz;
+  }
+}
+
+af_status diagnostics(const af_params& p, const Geometry& g, unsigned char* base, hipStream_t stream, double* energy,
+                      double* enstrophy, double* spectrum, int nshell) {
+  const int blocks = 512;
+  const size_t bytes = (size_t)2 * blocks * sizeof(double) + (size_t)(nshell > 0 ? nshell : 1) * sizeof(double);
+  double* scratch = nullptr;
+  hipError_t e = hipMalloc(&scratch, bytes);
+  if (e != hipSuccess) return fail(AF_ENOMEM, "diagnostics scratch");
+  double* part = scratch;
+  double* spec = scratch + 2 * blocks;
+  const double twopi = 6.283185307179586476925286766559;
+  const double tkx = twopi / p.lx, tky = twopi / p.ly;
+  const double kunit = tkx < tky ? tkx : tky;
+  const double N = (double)p.nx * (double)p.ny;
+  std::vector<double> hp(2 * blocks), hs(nshell > 0 ? nshell : 1);
+  MeanState ms;
+  af_status st = AF_OK;
+  do {
+    if ((e = hipMemsetAsync(spec, 0, (size_t)(nshell > 0 ? nshell : 1) * sizeof(double), stream)) != hipSuccess) {
+      st = hip_fail(e, "diag clear"); break;
+    }
+    if (g.prec == 64)
+      hipLaunchKernelGGL((diag_kernel<double>), dim3(blocks), dim3(256), 0, stream,
+                         (const cx<double>*)(base + g.off_field[0]), g.nx2, g.ny, g.by, g.bx, tkx, tky, kunit,
+                         1.0 / (N * N), part, spec, nshell);
+    else
+      hipLaunchKernelGGL((diag_kernel<float>), dim3(blocks), dim3(256), 0, stream,
+                         (const cx<float>*)(base + g.off_field[0]), g.nx2, g.ny, g.by, g.bx, tkx, tky, kunit,
+                         1.0 / (N * N), part, spec, nshell);
+    if ((e = hipGetLastError()) != hipSuccess) { st = hip_fail(e, "diag launch"); break; }
+    if ((e = hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, stream)) !=
+        hipSuccess) { st = hip_fail(e, "diag copy"); break; }
+    if ((e = hipMemcpyAsync(hs.data(), spec, hs.size() * sizeof(double), hipMemcpyDeviceToHost, stream)) !=
+        hipSuccess) { st = hip_fail(e, "diag copy"); break; }
+    if ((e = hipMemcpyAsync(&ms, base + g.off_mean, sizeof(ms), hipMemcpyDeviceToHost, stream)) != hipSuccess) {
+      st = hip_fail(e, "diag copy"); break;
+    }
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) { st = hip_fail(e, "diag sync"); break; }
+    double se = 0, sz = 0;
+    for (int b = 0; b < blocks; ++b) { se += hp[2 * b]; sz += hp[2 * b + 1]; }
+    const double emean = 0.5 * (ms.M[0] * ms.M[0] + ms.M[1] * ms.M[1]) / (N * N);   // k = 0: mean flow
+    if (energy) *energy = se + emean;
+    if (enstrophy) *enstrophy = sz;
+    if (spectrum && nshell > 0) {
+      for (int s = 0; s < nshell; ++s) spectrum[s] = hs[s];
+      spectrum[0] += emean;
+    }
+  } while (0);
+  (void)hipFree(scratch);
+  return st;
+}
+
+}  // namespace af

@@ -17,42 +17,9 @@
 #include <tuple>
 #include <vector>
 
-#include "run_impl.hpp"
+#include "engine_impl.hpp"
 
 using namespace af;
-
-namespace {
-
-struct Slot {
-  bool used = false;
-  af_params p;
-  Geometry g;
-  unsigned char* base = nullptr;
-  long long steps = 0;
-};
-
-struct Class {
-  hipStream_t stream = nullptr;
-  void* descs = nullptr;   // device array of per-run descriptors
-  int cap = 0;             // capacity of descs (entries)
-  size_t desc_bytes = 0;
-};
-
-using ClassKey = std::tuple<int, int, int, const Schedule*>;  // precision, nx, ny, schedule
-
-}  // namespace
-
-struct af_engine {
-  size_t budget = 0, in_use = 0;
-  std::vector<Slot> slots;
-  std::map<ClassKey, Class> classes;
-  hipStream_t stream = nullptr;
-  // cached graph for the last stepped slot list
-  std::vector<int> graph_slots;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  std::vector<hipEvent_t> events;
-};
 
 static void drop_graph(af_engine* e) {
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
@@ -251,6 +218,15 @@ af_status af_engine_get_energy(af_engine* e, int slot, int n, double* energy) {
   if (he != hipSuccess) return hip_fail(he, "energy download");
   for (int i = 0; i < n; ++i) energy[i] = ring[(size_t)((s.steps - n + i) % kRingCap)];
   return AF_OK;
+}
+
+af_status af_engine_diagnostics(af_engine* e, int slot, double* energy, double* enstrophy, double* spectrum,
+                                int nshell) {
+  if (!e || slot < 0 || slot >= (int)e->slots.size() || !e->slots[slot].used || nshell < 0 ||
+      (nshell > 0 && !spectrum))
+    return fail(AF_EINVAL, "bad arguments");
+  const Slot& s = e->slots[slot];
+  return diagnostics(s.p, s.g, s.base, e->stream, energy, enstrophy, spectrum, nshell);
 }
 
 long long af_engine_steps_done(const af_engine* e, int slot) {

@@ -36,6 +36,7 @@ struct InstDesc {
   int mean_scheme;    // 0 reference, 1 lawson
   int row_tiles;      // number of row blocks (partials entries)
   double dt, tkx, tky, c0, c2, c4, lam, beta, invN, e, e2;
+  double dt2, dt3, dt6;   // dt/2, dt/3, dt/6
 };
 
 }  // namespace af

@@ -141,6 +141,9 @@ static InstDesc<T> make_desc(const af_params* p, const Geometry& g, unsigned cha
   d.invN = 1.0 / ((double)p->nx * (double)p->ny);
   d.e = std::exp(p->c0 * p->dt);
   d.e2 = std::exp(p->c0 * p->dt * 0.5);
+  d.dt2 = p->dt / 2;
+  d.dt3 = p->dt / 3;
+  d.dt6 = p->dt / 6;
   return d;
 }
 
@@ -385,6 +388,12 @@ af_status af_run_step(af_run* r, int nsteps, double* energy, long long* nan_step
 }
 
 long long af_run_steps_done(const af_run* r) { return r ? r->steps : -1; }
+
+af_status af_run_diagnostics(af_run* r, double* energy, double* enstrophy, double* spectrum, int nshell) {
+  if (!r || nshell < 0 || (nshell > 0 && !spectrum)) return fail(AF_EINVAL, "bad arguments");
+  if (!r->ready) return fail(AF_EINVAL, "state not set");
+  return diagnostics(r->p, r->g, r->base, r->stream, energy, enstrophy, spectrum, nshell);
+}
 
 af_status af_run_use_graph(af_run* r, int on) {
   if (!r) return fail(AF_EINVAL, "run is NULL");

@@ -37,6 +37,10 @@
 
 namespace af {
 
+// 1/x: hardware reciprocal (1 ulp) in FP32, correctly rounded division in FP64.
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
 // Largest |ky index| kept by the mask in column x (-1: none), so the per-mode
 // test of the column pass is one integer compare.  Mode 0: the largest y with
 // 9 (x^2 Ny^2 + y^2 Nx^2) <= Nx^2 Ny^2; mode 1: 3y <= Ny when 3x <= Nx.
@@ -137,14 +141,15 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   const int j = t / RB;
   LdsView<T, NX2, RB, R> lds{lds_mem, yy};
 
-  // brick address of (row tile, position k) for this lane's row
-  const size_t tile_base = (size_t)tile * (NX2 / BX) * (RB * BX) + yy * BX;
-  auto addr = [&](int k) -> size_t { return tile_base + (size_t)(k / BX) * (RB * BX) + (k % BX); };
+  // brick offset of (row tile, position k) for this lane's row (fields hold
+  // at most 2^23 elements, so 32-bit offsets from a uniform base suffice)
+  const unsigned tile_base = (unsigned)tile * (NX2 / BX) * (RB * BX) + yy * BX;
+  auto addr = [&](int k) -> unsigned { return tile_base + (unsigned)(k / BX) * (RB * BX) + (k % BX); };
 
   cx<T> f3[3][R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const size_t a = addr(j + r * TR);
+    const unsigned a = addr(j + r * TR);
     f3[0][r] = d.um[a];
     f3[1][r] = d.vm[a];
     f3[2][r] = d.wm[a];
@@ -256,11 +261,12 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     }
   }
 
+  const T tkx = (T)d.tkx;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int k = j + r * TR;
-    const T kx = (T)(d.tkx * (double)k);
-    const size_t a = addr(k);
+    const T kx = tkx * (T)k;
+    const unsigned a = addr(k);
     d.pm[a] = cx<T>{kx * f2[0][r].im, -kx * f2[0][r].re};   // -i kx X1
     d.qm[a] = f2[1][r];
   }
@@ -300,11 +306,11 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
   const int nx2 = d.nx >> 1;
   LdsView<T, NY, W, R> lds{lds_mem, c};
 
-  const size_t col_base = (size_t)tile * (BY * W) + c;
-  const size_t brow = (size_t)(nx2 / W) * (BY * W);  // elements per brick row
-  auto addr = [&](int y) -> size_t { return col_base + (size_t)(y / BY) * brow + (y % BY) * W; };
+  const unsigned col_base = (unsigned)tile * (BY * W) + c;
+  const unsigned brow = (unsigned)(nx2 / W) * (BY * W);  // elements per brick row
+  auto addr = [&](int y) -> unsigned { return col_base + (unsigned)(y / BY) * brow + (y % BY) * W; };
 
-  const double dt = d.dt;
+  const double dt = d.dt, dt2 = d.dt2, dt3 = d.dt3, dt6 = d.dt6;
   const bool col0 = (x == 0);
   constexpr int RW = NT < 64 ? NT : 64;           // lanes of the first (possibly partial) wave
   const bool mean_wave = (tile == 0 && t < RW);   // owns the k = 0 mode and the mean ODE
@@ -327,7 +333,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
     cx<T> pq[2][R], wv[R], av[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const size_t a = addr(j + r * TC);
+      const unsigned a = addr(j + r * TC);
       pq[0][r] = d.pm[a];
       pq[1][r] = d.qm[a];
     }
@@ -343,7 +349,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
     cx<T> pw[R];   // projection partner (Ny - y) of the kx = 0 column
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const size_t a = addr(j + r * TC);
+      const unsigned a = addr(j + r * TC);
       if (stage <= 3) wv[r] = d.wo[a];
       if (stage >= 2) av[r] = d.acc[a];
       if (col0) {
@@ -400,16 +406,16 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
       }
       cx<T> s;
       if (stage == 1) {
-        a = a + cscale((T)(dt / 6 * (double)E), nh);
-        s = cscale(E2, wv[r] + cscale((T)(dt / 2), nh));
+        a = a + cscale((T)(dt6 * (double)E), nh);
+        s = cscale(E2, wv[r] + cscale((T)dt2, nh));
       } else if (stage == 2) {
-        a = a + cscale((T)(dt / 3 * (double)E2), nh);
-        s = cscale(E2, wv[r] + cscale((T)(dt / 2), nh));
+        a = a + cscale((T)(dt3 * (double)E2), nh);
+        s = cscale(E2, wv[r] + cscale((T)dt2, nh));
       } else if (stage == 3) {
-        a = a + cscale((T)(dt / 3 * (double)E2), nh);
+        a = a + cscale((T)(dt3 * (double)E2), nh);
         s = cscale(E, wv[r] + cscale((T)dt, nh));
       } else {
-        a = a + cscale((T)(dt / 6), nh);
+        a = a + cscale((T)dt6, nh);
         s = a;
       }
       an[r] = a;
@@ -440,10 +446,10 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-          if (stage == 1) { M[i] += dt / 6 * d.e * g[i];  mm[i] = d.e2 * (Mo[i] + dt / 2 * g[i]); }
-          else if (stage == 2) { M[i] += dt / 3 * d.e2 * g[i]; mm[i] = d.e2 * (Mo[i] + dt / 2 * g[i]); }
-          else if (stage == 3) { M[i] += dt / 3 * d.e2 * g[i]; mm[i] = d.e * (Mo[i] + dt * g[i]); }
-          else { M[i] += dt / 6 * g[i]; mm[i] = M[i]; }
+          if (stage == 1) { M[i] += dt6 * d.e * g[i];  mm[i] = d.e2 * (Mo[i] + dt2 * g[i]); }
+          else if (stage == 2) { M[i] += dt3 * d.e2 * g[i]; mm[i] = d.e2 * (Mo[i] + dt2 * g[i]); }
+          else if (stage == 3) { M[i] += dt3 * d.e2 * g[i]; mm[i] = d.e * (Mo[i] + dt * g[i]); }
+          else { M[i] += dt6 * g[i]; mm[i] = M[i]; }
         }
         ms->M[0] = M[0]; ms->M[1] = M[1];
         ms->Mo[0] = Mo[0]; ms->Mo[1] = Mo[1];
@@ -463,15 +469,16 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
   // v_x^ = i ky s^/k^2, v_y^ = -i kx s^/k^2 (nonl_subfunc_0, .cu:1176-1180),
   // scaled by 1/N so the row pass sees physical fields.
   T fx[R], fy[R];
+  const T kxT = (T)kxd, tkyT = (T)d.tky, invNT = (T)d.invN;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int y = j + r * TC;
     const int nys = (y <= NY / 2) ? y : y - NY;
-    const double kyd = d.tky * (double)nys;
-    const double k2 = kx2 + kyd * kyd;
-    const double ik = (k2 > 0.0) ? d.invN / k2 : 0.0;
-    fx[r] = (T)(kyd * ik);
-    fy[r] = (T)(-kxd * ik);
+    const T ky = tkyT * (T)nys;
+    const T k2 = kxT * kxT + ky * ky;
+    const T ik = (k2 > (T)0) ? invNT * recip(k2) : (T)0;
+    fx[r] = ky * ik;
+    fy[r] = -kxT * ik;
   }
   const T invN = (T)d.invN;
   auto make = [&](cx<T>(&q)[R], int f) {
@@ -494,7 +501,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
     fft_multi<NY, R, +1, 3>(q3, j, lds, tw_y);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const size_t a = addr(j + r * TC);
+      const unsigned a = addr(j + r * TC);
       d.um[a] = q3[0][r];
       d.vm[a] = q3[1][r];
       d.wm[a] = q3[2][r];

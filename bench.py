@@ -88,15 +88,82 @@ def roofline(r):
     return name, alg, ms, achieved
 
 
-def cpu_baseline(n=1024, prec=32, steps=3):
-    """The oracle (C++ restatement of the reference step) on the host cores."""
+def ensemble_leg(af, steps=200, seed=7):
+    """Config 4: 64 instances (32 x 256^2, 24 x 512^2, 8 x 1024^2), seeded random
+    alpha, beta, lambda0, FP32, stepped together in shared launches; compared
+    with the same instances run one after another through the single-run path
+    (timed per size class and summed)."""
+    rng = np.random.default_rng(seed)
+    mix = [(256, 32), (512, 24), (1024, 8)]
+    eng = af.Engine(200 << 30)
+    pts = 0
+    for n, cnt in mix:
+        for i in range(cnt):
+            p = af.params(n, dt=2e-4 if n <= 256 else 1e-4, lam=float(rng.uniform(1, 5)),
+                          alpha=float(rng.uniform(-1.2, -0.4)), beta=float(rng.uniform(0.005, 0.02)),
+                          gamma0=-0.045, gamma2=2.5e-5, precision=32)
+            eng.add(p, af.ic_bandlimited(n, seed=int(rng.integers(1, 1 << 30)), amp=1e-3, kmax=10.0))
+            pts += n * n
+    eng.step(10)
+    t0 = time.perf_counter()
+    eng.step(steps)
+    wall = time.perf_counter() - t0
+    value = pts * steps / wall
+    eng.close()
+    seq = 0.0
+    for n, cnt in mix:
+        r = run_workload(af, n, 32, max(50, steps // 2), 10)
+        seq += cnt * (r["ms_per_step"] * 1e-3) * steps
+    fb = sum(cnt * field_bytes(n, 32) for n, cnt in mix)
+    return {"instances": 64, "mix": "32x256^2 + 24x512^2 + 8x1024^2 FP32", "steps": steps,
+            "value": value, "ms_per_step": wall / steps * 1e3, "sequential_value": pts * steps / seq,
+            "speedup_vs_sequential": seq / wall,
+            "step_bytes_50_transfers": 50 * fb,
+            "hbm_frac_50_transfers": 50 * fb / (wall / steps) / 1e9 / HBM_PEAK_GBS}
+
+
+def serving_leg(af, rate, window=0.5, seed=11, horizon_scale=0.1):
+    """Config 5: Poisson arrivals at `rate`/s for `window` s; N from
+    {128..2048}^2 with weights 0.3/0.3/0.2/0.15/0.05; horizon U{1k..50k}
+    scaled by horizon_scale; snapshot every 500 steps; 200 GB budget."""
+    rng = np.random.default_rng(seed)
+    sizes, wts = [128, 256, 512, 1024, 2048], [0.3, 0.3, 0.2, 0.15, 0.05]
+    reqs, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > window:
+            break
+        n = int(rng.choice(sizes, p=wts))
+        p = af.params(n, dt=2e-4 if n <= 256 else (1e-4 if n <= 1024 else 5e-5), lam=float(rng.uniform(1, 5)),
+                      alpha=float(rng.uniform(-1.2, -0.4)), beta=float(rng.uniform(0.005, 0.02)),
+                      gamma0=-0.045, gamma2=2.5e-5, precision=32)
+        h = int(rng.integers(1000, 50001) * horizon_scale)
+        reqs.append(af.Request(arrival_s=t, params=p, horizon=h, snapshot_every=500,
+                               seed=int(rng.integers(1, 1 << 30))))
+    nsnap = [0]
+    res, st, _ = af.serve(reqs, hbm_budget=200 << 30, max_tick_steps=50,
+                          on_snapshot=lambda *a: nsnap.__setitem__(0, nsnap[0] + 1))
+    return {"rate_per_s": rate, "window_s": window, "requests": len(reqs), "completed": st["completed"],
+            "horizon": f"U{{1k..50k}} x {horizon_scale}", "throughput": st["throughput"],
+            "step_p50_ms": st["step_p50_s"] * 1e3, "step_p99_ms": st["step_p99_s"] * 1e3,
+            "ttc_p50_s": st["ttc_p50_s"], "ttc_p99_s": st["ttc_p99_s"], "queue_p50_s": st["queue_p50_s"],
+            "queue_p99_s": st["queue_p99_s"], "max_resident": st["max_resident"], "snapshots": nsnap[0],
+            "wall_s": st["wall_s"]}
+
+
+def cpu_baseline(n=1024, prec=32, target_s=10.0):
+    """The oracle (C++ restatement of the reference step) on the host cores,
+    timed over enough steps for ~target_s of CPU work (at least 3)."""
     from oracle.oracle_py import Oracle
     af = importlib.import_module("2d_active_matter_simulation_amd")
     threads = min(os.cpu_count() or 1, 16)
     o = Oracle(n, dt=1e-4, lam=PHYS["lam"], beta=PHYS["beta"], c0=-PHYS["alpha"], c2=-PHYS["gamma0"],
                c4=-PHYS["gamma2"], precision=prec, nthreads=threads)
     o.set_state(af.ic_bandlimited(n, seed=1, amp=1e-3, kmax=10.0))
+    t0 = time.perf_counter()
     o.step(1)
+    first = time.perf_counter() - t0
+    steps = max(3, int(target_s / max(first, 1e-3)))
     t0 = time.perf_counter()
     o.step(steps, energy=True)
     wall = time.perf_counter() - t0
@@ -112,6 +179,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip FP64 / 2048^2 / 4096^2 lines")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-instance ensemble leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the served-workload leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -162,6 +231,10 @@ def main():
                      "step_model_frac": 76 * 1024 * 513 * 8 / (head["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "extra": extra,
     }
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_leg(af)
+    if not args.no_serving:
+        out["serving"] = [serving_leg(af, r) for r in (10, 50, 200)]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

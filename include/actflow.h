@@ -85,6 +85,12 @@ af_status af_run_get_state(af_run* r, double* omega_hat, double M[2]);
 af_status af_run_step(af_run* r, int nsteps, double* energy, long long* nan_step);
 /* Steps completed since set_state. */
 long long af_run_steps_done(const af_run* r);
+/* Diagnostics of the current state (no reference counterpart; the reference
+ * writes only the energy, .cu:1118-1127): energy 1/2<|u|^2> including the
+ * mean flow, enstrophy 1/2<omega^2>, and (if nshell > 0) the shell spectrum
+ * spectrum[s] = sum of 1/2|u^(k)|^2/N^2 over |k|/k_unit in [s-1/2, s+1/2),
+ * k_unit = min(2pi/Lx, 2pi/Ly); the mean flow is in shell 0. */
+af_status af_run_diagnostics(af_run* r, double* energy, double* enstrophy, double* spectrum, int nshell);
 void af_run_destroy(af_run* r);
 
 /* Per-kernel timing with HIP events on the run's stream: runs nsteps steps
@@ -134,9 +140,49 @@ af_status af_engine_get_state(af_engine* e, int slot, double* omega_hat, double 
 /* Copies the last n per-step energies of a slot (oldest first). */
 af_status af_engine_get_energy(af_engine* e, int slot, int n, double* energy);
 long long af_engine_steps_done(const af_engine* e, int slot);
+af_status af_engine_diagnostics(af_engine* e, int slot, double* energy, double* enstrophy, double* spectrum,
+                                int nshell);
 size_t af_engine_bytes_in_use(const af_engine* e);
 int af_engine_count(const af_engine* e);
 void af_engine_destroy(af_engine* e);
+
+/* ---- serving loop (no reference counterpart) -------------------------- */
+typedef struct {
+  double arrival_s;          /* arrival, seconds after af_serve starts           */
+  af_params params;
+  int horizon;               /* steps to run                                     */
+  int snapshot_every;        /* steps between snapshots, 0 = none                */
+  unsigned long long seed;   /* band-limited IC (af_ic_bandlimited)              */
+  double ic_amp, ic_kmax;
+} af_request;
+typedef struct {
+  double admit_s, finish_s;          /* seconds after start; -1 if never       */
+  double step_p50_s, step_p99_s;     /* interval between consecutive steps     */
+  int snapshots;                     /* snapshots delivered                    */
+  int status;                        /* af_status of this request              */
+  double M[2];                       /* final mean velocity coefficient        */
+} af_request_result;
+typedef struct {
+  double wall_s, grid_point_steps, throughput;   /* throughput: pt-steps/s      */
+  double step_p50_s, step_p99_s;     /* over every step of every request        */
+  double ttc_p50_s, ttc_p99_s;       /* time to completion, finish - arrival    */
+  double queue_p50_s, queue_p99_s;   /* admission wait, admit - arrival         */
+  int max_resident, ticks, completed;
+  size_t peak_bytes;                 /* peak run-state bytes held               */
+} af_serve_stats;
+/* Called on the calling thread for every snapshot, in completion order;
+ * omega_hat is in the reference layout (valid during the call only). */
+typedef void (*af_snapshot_fn)(void* user, int request, long long step, const double* omega_hat,
+                               const double M[2], int nx, int ny);
+/* Runs the requests with continuous batching: admission in arrival order by
+ * the HBM budget, all resident runs advanced together in ticks of at most
+ * max_tick_steps (ticks end at snapshot and completion steps), snapshots
+ * copied back asynchronously, finished runs retired so waiting requests can
+ * enter.  final_states (optional) holds per request NULL or a buffer of
+ * 2*Ny*(Nx/2+1) doubles for the final state.  A request's result is bitwise
+ * the result of running it alone. */
+af_status af_serve(const af_request* reqs, int n, size_t hbm_budget, int max_tick_steps, af_snapshot_fn fn,
+                   void* user, double** final_states, af_request_result* results, af_serve_stats* stats);
 
 /* ---- host formats (reference file contract) --------------------------- */
 typedef struct {
