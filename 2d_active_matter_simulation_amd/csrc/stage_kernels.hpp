@@ -37,6 +37,27 @@
 
 namespace af {
 
+// Pointers read from an instance descriptor are generic; the fields live in
+// device global memory, so accesses go through an address-space-1 view (global
+// instructions with a scalar base and a 32-bit lane offset instead of FLAT).
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T> using gptr = __attribute__((address_space(1))) T*;
+#else   // host pass of hipcc: these functions are never executed on the host
+template <class T> using gptr = T*;
+#endif
+template <class T> __device__ __forceinline__ gptr<T> glob(T* p) { return (gptr<T>)p; }
+
+// Element idx of a global array as a 32-bit byte offset from the (uniform)
+// base, the form the scalar-base addressing of global loads takes.
+template <class T>
+struct Field {
+  gptr<char> b;
+  __device__ __forceinline__ explicit Field(T* p) : b((gptr<char>)glob(p)) {}
+  __device__ __forceinline__ gptr<T> at(unsigned idx) const { return (gptr<T>)(b + idx * (unsigned)sizeof(T)); }
+  __device__ __forceinline__ T operator[](unsigned idx) const { return *at(idx); }
+  __device__ __forceinline__ void put(unsigned idx, T v) const { *at(idx) = v; }
+};
+
 // 1/x: hardware reciprocal (1 ulp) in FP32, correctly rounded division in FP64.
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
@@ -45,22 +66,18 @@ __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
 // test of the column pass is one integer compare.  Mode 0: the largest y with
 // 9 (x^2 Ny^2 + y^2 Nx^2) <= Nx^2 Ny^2; mode 1: 3y <= Ny when 3x <= Nx.
 __device__ __forceinline__ int mask_ymax(int x, int nx, int ny, int mode) {
-  const long long X = x, NX = nx, NY = ny;
-  if (mode != 0) return (3 * X <= NX) ? (int)(NY / 3) : -1;
-  const long long rhs = NY * NY * (NX * NX - 9 * X * X);   // >= 9 NX^2 y^2
+  if (mode != 0) return (3 * x <= nx) ? ny / 3 : -1;
+  // all products below are integers < 2^53, exact in double
+  const double NX = nx, NY = ny, X = x;
+  const double rhs = NY * NY * (NX * NX - 9.0 * X * X);   // >= 9 NX^2 y^2
   if (rhs < 0) return -1;
-  const long long c = 9 * NX * NX;
-  long long y = (long long)sqrt((double)rhs / (double)c);
-  while (y > 0 && c * y * y > rhs) --y;
-  while (c * (y + 1) * (y + 1) <= rhs) ++y;
+  const double c = 9.0 * NX * NX;
+  double y = floor(sqrt(rhs / c));
+  if (c * (y + 1) * (y + 1) <= rhs) y += 1;
+  if (c * y * y > rhs) y -= 1;
   return (int)y;
 }
 
-__device__ __forceinline__ bool keep_mode(int x, int yabs, int nx, int ny, int mode) {
-  const long long X = x, Y = yabs, NX = nx, NY = ny;
-  if (mode == 0) return 9 * (X * X * NY * NY + Y * Y * NX * NX) <= NX * NX * NY * NY;
-  return 3 * X <= NX && 3 * Y <= NY;
-}
 
 // LDS view of NF field areas, each holding W interleaved transforms of
 // length L (the W columns of a column block or the RB rows of a row block).
@@ -85,7 +102,8 @@ __device__ __forceinline__ const cx<T>* tw_of(const cx<T>* pool, int L) { return
 // Copies n twiddles into LDS (caller places a barrier before their first use).
 template <class T, int NT>
 __device__ __forceinline__ void stage_twiddles(cx<T>* dst, const cx<T>* __restrict__ src, int n) {
-  for (int i = threadIdx.x; i < n; i += NT) dst[i] = src[i];
+  const gptr<const cx<T>> g = glob(src);
+  for (int i = threadIdx.x; i < n; i += NT) dst[i] = g[i];
 }
 
 // Deterministic block sum of NV doubles; result valid in thread 0.
@@ -146,13 +164,14 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   const unsigned tile_base = (unsigned)tile * (NX2 / BX) * (RB * BX) + yy * BX;
   auto addr = [&](int k) -> unsigned { return tile_base + (unsigned)(k / BX) * (RB * BX) + (k % BX); };
 
+  const Field<cx<T>> gum(d.um), gvm(d.vm), gwm(d.wm);
   cx<T> f3[3][R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const unsigned a = addr(j + r * TR);
-    f3[0][r] = d.um[a];
-    f3[1][r] = d.vm[a];
-    f3[2][r] = d.wm[a];
+    f3[0][r] = gum[a];
+    f3[1][r] = gvm[a];
+    f3[2][r] = gwm[a];
   }
   stage_twiddles<T, NT>(tw_h, tw_of(pool, NX2), NX2);
   stage_twiddles<T, NT>(tw_n, tw_of(pool, 2 * NX2), NX2);
@@ -267,8 +286,8 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     const int k = j + r * TR;
     const T kx = tkx * (T)k;
     const unsigned a = addr(k);
-    d.pm[a] = cx<T>{kx * f2[0][r].im, -kx * f2[0][r].re};   // -i kx X1
-    d.qm[a] = f2[1][r];
+    Field<cx<T>>(d.pm).put(a, cx<T>{kx * f2[0][r].im, -kx * f2[0][r].re});   // -i kx X1
+    Field<cx<T>>(d.qm).put(a, f2[1][r]);
   }
 
   double sums[3] = {esum, gxs, gys};
@@ -322,7 +341,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
 
   if (stage == 0) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) sv[r] = d.wo[addr(j + r * TC)];
+    for (int r = 0; r < R; ++r) sv[r] = Field<cx<T>>(d.wo)[addr(j + r * TC)];
     stage_twiddles<T, NT>(tw_y, tw_of(pool, NY), NY);
     if (tile == 0 && t == 0) {
       minj[0] = d.mean->M[0];
@@ -334,27 +353,28 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const unsigned a = addr(j + r * TC);
-      pq[0][r] = d.pm[a];
-      pq[1][r] = d.qm[a];
+      pq[0][r] = Field<cx<T>>(d.pm)[a];
+      pq[1][r] = Field<cx<T>>(d.qm)[a];
     }
     // row-block partials for the mean ODE, summed per lane in a fixed order
     double s0 = 0, s1 = 0, s2 = 0;
     if (mean_wave) {
       for (int i = t; i < d.row_tiles; i += RW) {
-        s0 += d.partials[(size_t)i * 4 + 0];
-        s1 += d.partials[(size_t)i * 4 + 1];
-        s2 += d.partials[(size_t)i * 4 + 2];
+        const gptr<double> pp = glob(d.partials);
+        s0 += pp[(unsigned)i * 4 + 0];
+        s1 += pp[(unsigned)i * 4 + 1];
+        s2 += pp[(unsigned)i * 4 + 2];
       }
     }
     cx<T> pw[R];   // projection partner (Ny - y) of the kx = 0 column
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const unsigned a = addr(j + r * TC);
-      if (stage <= 3) wv[r] = d.wo[a];
-      if (stage >= 2) av[r] = d.acc[a];
+      if (stage <= 3) wv[r] = Field<cx<T>>(d.wo)[a];
+      if (stage >= 2) av[r] = Field<cx<T>>(d.acc)[a];
       if (col0) {
         const int yp = (NY - (j + r * TC)) & (NY - 1);
-        pw[r] = (stage == 1) ? d.wo[addr(yp)] : d.acc[addr(yp)];
+        pw[r] = Field<cx<T>>((stage == 1) ? d.wo : d.acc)[addr(yp)];
       }
     }
     double Mcur[2] = {0, 0}, Mold[2] = {0, 0};
@@ -422,9 +442,9 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
       sv[r] = s;
     }
     __syncthreads();   // every projection-partner read precedes the writes below
-    cx<T>* dst = (stage == 4) ? d.wo : d.acc;
+    const Field<cx<T>> dst((stage == 4) ? d.wo : d.acc);
 #pragma unroll
-    for (int r = 0; r < R; ++r) dst[addr(j + r * TC)] = an[r];
+    for (int r = 0; r < R; ++r) dst.put(addr(j + r * TC), an[r]);
 
     // Mean-velocity ODE (host updates of .cu:729-821), FP64, tile 0 only.
     if (mean_wave) {
@@ -502,9 +522,9 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const unsigned a = addr(j + r * TC);
-      d.um[a] = q3[0][r];
-      d.vm[a] = q3[1][r];
-      d.wm[a] = q3[2][r];
+      Field<cx<T>>(d.um).put(a, q3[0][r]);
+      Field<cx<T>>(d.vm).put(a, q3[1][r]);
+      Field<cx<T>>(d.wm).put(a, q3[2][r]);
     }
   } else {
 #pragma unroll
@@ -513,9 +533,9 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
       make(q[0], f);
       __syncthreads();
       fft_multi<NY, R, +1, 1>(q, j, lds, tw_y);
-      cx<T>* dst = (f == 0) ? d.um : (f == 1) ? d.vm : d.wm;
+      const Field<cx<T>> dst((f == 0) ? d.um : (f == 1) ? d.vm : d.wm);
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[addr(j + r * TC)] = q[0][r];
+      for (int r = 0; r < R; ++r) dst.put(addr(j + r * TC), q[0][r]);
     }
   }
 }
