@@ -90,97 +90,112 @@ struct Dft {
 constexpr int ilog(int n, int b) { return n < b ? 0 : 1 + ilog(n / b, b); }
 constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 
-// NF length-L transforms done together (one LDS exchange per radix pass for
-// all of them).  `Lds` provides load(f, p, xs) / store(f, p, v, xs) for
-// positions p in [0, L) of field f of this lane's transform, where xs is the
-// stride Ns of the radix pass that stores (so each exchange can use its own
-// bank-conflict-free padding).  tw points at
-// tw[m] = exp(-2 pi i m / L), m in [0, L) (normally an LDS copy).  All lanes
-// of the block must call this together (it contains block barriers).  The
-// first LDS access is a store after radix pass 1, so the caller must place a
-// barrier before the call if other lanes may still be reading the same LDS
-// areas from an earlier use.
-template <int L, int R, int DIR, int NF, class T, class Lds>
-__device__ __forceinline__ void fft_multi(cx<T> (&v)[NF][R], int j, Lds& lds, const cx<T>* tw) {
-  static_assert((L & (L - 1)) == 0 && (R & (R - 1)) == 0 && R <= L && R <= 16, "power-of-two sizes");
-  constexpr int NP = ilog(L, R);           // full radix-R passes
-  constexpr int RL = L / ipow(R, NP);      // trailing radix (1 if L is a power of R)
-  constexpr int S = L / R;                 // lanes per transform
-  int Ns = 1;
-#pragma unroll
-  for (int p = 0; p < NP; ++p) {
-    if (p > 0) {
-      __syncthreads();
-#pragma unroll
-      for (int f = 0; f < NF; ++f)
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[f][r] = lds.load(f, j + r * S, Ns / R);
-      const int jm = j & (Ns - 1);
-      const int stride = L / (Ns * R);
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        const cx<T> w = tw[r * jm * stride];
-#pragma unroll
-        for (int f = 0; f < NF; ++f) v[f][r] = DIR < 0 ? cmul(v[f][r], w) : cmulc(v[f][r], w);
-      }
-    }
-#pragma unroll
-    for (int f = 0; f < NF; ++f) Dft<R, DIR, T>::run(v[f]);
-    const bool last = (p == NP - 1) && (RL == 1);
-    if (!last) {
-      if (p > 0) __syncthreads();
-      const int base = (j / Ns) * Ns * R + (j & (Ns - 1));
-#pragma unroll
-      for (int f = 0; f < NF; ++f)
-#pragma unroll
-        for (int r = 0; r < R; ++r) lds.store(f, base + r * Ns, v[f][r], Ns);
-    }
-    Ns *= R;
-  }
-  if constexpr (RL > 1) {
-    __syncthreads();
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-#pragma unroll
-      for (int m = 0; m < R; ++m) v[f][m] = lds.load(f, j + m * S, Ns / R);
-    // R/RL butterflies of radix RL on elements m = q + (R/RL) r'
-#pragma unroll
-    for (int q = 0; q < R / RL; ++q) {
-      const int b = j + q * S;
-      cx<T> w[RL];
-#pragma unroll
-      for (int rr = 1; rr < RL; ++rr) w[rr] = tw[rr * b];
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        cx<T> u[RL];
-#pragma unroll
-        for (int rr = 0; rr < RL; ++rr) {
-          cx<T> x = v[f][q + (R / RL) * rr];
-          if (rr > 0) x = DIR < 0 ? cmul(x, w[rr]) : cmulc(x, w[rr]);
-          u[rr] = x;
-        }
-        Dft<RL, DIR, T>::run(u);
-#pragma unroll
-        for (int rr = 0; rr < RL; ++rr) v[f][q + (R / RL) * rr] = u[rr];
-      }
-    }
-  }
-}
-
 // LDS index of element p of column c when W transforms are interleaved
 // (i = p*W + c), padded for the exchange written by a radix-R pass of stride
 // Ns: Ns*W extra elements every Ns*R*W elements, so that the groups of a
 // stride-Ns store land in disjoint bank ranges; no padding once a contiguous
 // run of Ns*W elements spans all 64 four-byte banks.  Conflict-free for the
 // store/load pairs of fft_multi (checked against a 32- and 64-bank model).
+// For a given exchange every lane's R stores and R loads sit at a lane base
+// plus offsets that do not depend on the lane, so fft_multi computes one base
+// per lane and uses the offsets as immediates.
 template <int W, int R, int ESZ>
-__device__ __forceinline__ int lds_pad_index(int p, int c, int Ns) {
-  const int i = p * W + c;
-  if (Ns * W * ESZ >= 256) return i;
-  const int g = Ns * R * W;            // power of two
-  return i + (i / g) * (Ns * W);
+__host__ __device__ constexpr int lds_pad(int i, int Ns) {
+  return (Ns * W * ESZ >= 256) ? i : i + (i / (Ns * R * W)) * (Ns * W);
 }
 // Elements of one LDS area for transforms of length L (covers every padding).
 template <int L, int W, int R> constexpr int lds_area_elems() { return L * W + (L * W) / R + 1; }
+
+// NF length-L transforms done together (one LDS exchange per radix pass for
+// all of them).  lds points at this block's NF areas (lds_area_elems each) of
+// W interleaved transforms; this lane is transform lane j of column c.  tw
+// points at tw[m] = exp(-2 pi i m / L), m in [0, L) (normally an LDS copy).
+// All lanes of the block must call this together (it contains block
+// barriers).  The first LDS access is a store after radix pass 1, so the
+// caller must place a barrier before the call if other lanes may still be
+// reading the same LDS areas from an earlier use.
+template <int L, int R, int DIR, int NF, int W, class T>
+struct FftMulti {
+  static_assert((L & (L - 1)) == 0 && (R & (R - 1)) == 0 && R <= L && R <= 16, "power-of-two sizes");
+  static constexpr int NP = ilog(L, R);       // full radix-R passes
+  static constexpr int RL = L / ipow(R, NP);  // trailing radix (1 if L is a power of R)
+  static constexpr int S = L / R;             // lanes per transform
+  static constexpr int ESZ = (int)sizeof(cx<T>);
+  static constexpr int AREA = lds_area_elems<L, W, R>();
+  static __device__ __forceinline__ int pad(int i, int Ns) { return lds_pad<W, R, ESZ>(i, Ns); }
+
+  // loads of the positions j + r*S written by the pass of stride Nsp
+  template <int Nsp>
+  static __device__ __forceinline__ void load(cx<T> (&v)[NF][R], int j, int c, const cx<T>* lds) {
+    const int a = pad(j * W + c, Nsp);
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[f][r] = lds[f * AREA + a + lds_pad<W, R, ESZ>(r * S * W, Nsp)];
+  }
+
+  template <int P>
+  static __device__ __forceinline__ void pass(cx<T> (&v)[NF][R], int j, int c, cx<T>* lds, const cx<T>* tw) {
+    if constexpr (P < NP) {
+      constexpr int Ns = ipow(R, P);
+      if constexpr (P > 0) {
+        __syncthreads();
+        load<Ns / R>(v, j, c, lds);
+        const int jm = j & (Ns - 1);
+        const int tj = jm * (L / (Ns * R));
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const cx<T> w = tw[r * tj];
+#pragma unroll
+          for (int f = 0; f < NF; ++f) v[f][r] = DIR < 0 ? cmul(v[f][r], w) : cmulc(v[f][r], w);
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < NF; ++f) Dft<R, DIR, T>::run(v[f]);
+      constexpr bool last = (P == NP - 1) && (RL == 1);
+      if constexpr (!last) {
+        if constexpr (P > 0) __syncthreads();
+        const int base = (j / Ns) * Ns * R + (j & (Ns - 1));
+        const int a = pad(base * W + c, Ns);
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+          for (int r = 0; r < R; ++r) lds[f * AREA + a + r * Ns * W] = v[f][r];
+      }
+      pass<P + 1>(v, j, c, lds, tw);
+    } else if constexpr (RL > 1) {
+      constexpr int Nsp = ipow(R, NP - 1);
+      __syncthreads();
+      load<Nsp>(v, j, c, lds);
+      // R/RL butterflies of radix RL on elements m = q + (R/RL) r'
+#pragma unroll
+      for (int q = 0; q < R / RL; ++q) {
+        const int b = j + q * S;
+        cx<T> w[RL];
+#pragma unroll
+        for (int rr = 1; rr < RL; ++rr) w[rr] = tw[rr * b];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          cx<T> u[RL];
+#pragma unroll
+          for (int rr = 0; rr < RL; ++rr) {
+            cx<T> x = v[f][q + (R / RL) * rr];
+            if (rr > 0) x = DIR < 0 ? cmul(x, w[rr]) : cmulc(x, w[rr]);
+            u[rr] = x;
+          }
+          Dft<RL, DIR, T>::run(u);
+#pragma unroll
+          for (int rr = 0; rr < RL; ++rr) v[f][q + (R / RL) * rr] = u[rr];
+        }
+      }
+    }
+  }
+};
+
+template <int L, int R, int DIR, int NF, class T, class Lds>
+__device__ __forceinline__ void fft_multi(cx<T> (&v)[NF][R], int j, Lds& lds, const cx<T>* tw) {
+  static_assert(Lds::kL == L && Lds::kR == R, "LDS areas laid out for this transform");
+  FftMulti<L, R, DIR, NF, Lds::kW, T>::template pass<0>(v, j, lds.c, lds.base, tw);
+}
 
 }  // namespace af

@@ -14,8 +14,17 @@ hipError_t launch_row(const void* descs, int ninst, int tiles, const void* pool,
 
 template <class T, int NY, int W, int R, int BY, int GF, int GI>
 hipError_t launch_col(const void* descs, int ninst, int tiles, int stage, const void* pool, hipStream_t s) {
-  hipLaunchKernelGGL((col_pass<T, NY, W, R, BY, GF, GI>), dim3(ninst * tiles), dim3(W * (NY / R)), 0, s,
-                     (const InstDesc<T>*)descs, tiles, stage, (const cx<T>*)pool);
+  const dim3 grid(ninst * tiles), block(W * (NY / R));
+  const InstDesc<T>* dd = (const InstDesc<T>*)descs;
+  const cx<T>* pp = (const cx<T>*)pool;
+  switch (stage) {
+    case 0: hipLaunchKernelGGL((col_pass<T, NY, W, R, BY, GF, GI, 0>), grid, block, 0, s, dd, tiles, pp); break;
+    case 1: hipLaunchKernelGGL((col_pass<T, NY, W, R, BY, GF, GI, 1>), grid, block, 0, s, dd, tiles, pp); break;
+    case 2: hipLaunchKernelGGL((col_pass<T, NY, W, R, BY, GF, GI, 2>), grid, block, 0, s, dd, tiles, pp); break;
+    case 3: hipLaunchKernelGGL((col_pass<T, NY, W, R, BY, GF, GI, 3>), grid, block, 0, s, dd, tiles, pp); break;
+    case 4: hipLaunchKernelGGL((col_pass<T, NY, W, R, BY, GF, GI, 4>), grid, block, 0, s, dd, tiles, pp); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

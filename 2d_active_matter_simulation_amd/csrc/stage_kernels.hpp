@@ -53,9 +53,17 @@ template <class T>
 struct Field {
   gptr<char> b;
   __device__ __forceinline__ explicit Field(T* p) : b((gptr<char>)glob(p)) {}
+  __device__ __forceinline__ Field(const Field&) = default;
   __device__ __forceinline__ gptr<T> at(unsigned idx) const { return (gptr<T>)(b + idx * (unsigned)sizeof(T)); }
   __device__ __forceinline__ T operator[](unsigned idx) const { return *at(idx); }
   __device__ __forceinline__ void put(unsigned idx, T v) const { *at(idx) = v; }
+  // the same array viewed from a uniform element offset further on (keeps
+  // per-row-group offsets in the scalar base instead of one VGPR per group)
+  __device__ __forceinline__ Field plus(unsigned u) const {
+    Field f(*this);
+    f.b = b + (size_t)u * sizeof(T);
+    return f;
+  }
 };
 
 // 1/x: hardware reciprocal (1 ulp) in FP32, correctly rounded division in FP64.
@@ -78,18 +86,13 @@ __device__ __forceinline__ int mask_ymax(int x, int nx, int ny, int mode) {
   return (int)y;
 }
 
-
 // LDS view of NF field areas, each holding W interleaved transforms of
 // length L (the W columns of a column block or the RB rows of a row block).
 template <class T, int L, int W, int R>
 struct LdsView {
+  static constexpr int kL = L, kW = W, kR = R;
   cx<T>* base;
   int c;
-  __device__ __forceinline__ int a(int f, int p, int xs) const {
-    return f * lds_area_elems<L, W, R>() + lds_pad_index<W, R, (int)sizeof(cx<T>)>(p, c, xs);
-  }
-  __device__ __forceinline__ cx<T> load(int f, int p, int xs) const { return base[a(f, p, xs)]; }
-  __device__ __forceinline__ void store(int f, int p, cx<T> v, int xs) { base[a(f, p, xs)] = v; }
   // unpadded access for the even/odd pair exchange of the real transforms
   __device__ __forceinline__ cx<T> load_plain(int f, int p) const { return base[f * lds_area_elems<L, W, R>() + p * W + c]; }
   __device__ __forceinline__ void store_plain(int f, int p, cx<T> v) { base[f * lds_area_elems<L, W, R>() + p * W + c] = v; }
@@ -303,18 +306,21 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 // ---------------------------------------------------------------------------
 // Column pass.  Block = W adjacent columns x TC = NY/R lanes per column.
 // stage 0: initial pass only (v^ from the state, inverse transforms).
-// stage 1..4: full stage.  GF = 2 / GI = 3: the forward (P, Q) and inverse
+// stage 1..4: full stage.  The stage is a template argument: each stage's
+// kernel keeps only the state it reads live (fewer registers, more waves).  GF = 2 / GI = 3: the forward (P, Q) and inverse
 // (v_x, v_y, omega) transforms share LDS exchanges; 1: one at a time.
 // ---------------------------------------------------------------------------
-template <class T, int NY, int W, int R, int BY, int GF, int GI>
+template <class T, int NY, int W, int R, int BY, int GF, int GI, int stage>
 __global__ void __launch_bounds__(W * (NY / R))
-col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, const cx<T>* __restrict__ pool) {
+col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>* __restrict__ pool) {
   constexpr int TC = NY / R;
   constexpr int NT = W * TC;
   constexpr int GMAX = GF > GI ? GF : GI;
   static_assert((GF == 1 || GF == 2) && (GI == 1 || GI == 3), "field groups");
   __shared__ cx<T> lds_mem[GMAX * lds_area_elems<NY, W, R>()];
   __shared__ cx<T> tw_y[NY];
+  __shared__ double mean_in[7];   // tile 0: sum|u|^2, sum Cx, sum Cy, M, M_old
+  __shared__ long long nstep_in;
 
   const int inst = blockIdx.x / tiles_per_inst;
   const int tile = blockIdx.x - inst * tiles_per_inst;
@@ -328,6 +334,12 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
   const unsigned col_base = (unsigned)tile * (BY * W) + c;
   const unsigned brow = (unsigned)(nx2 / W) * (BY * W);  // elements per brick row
   auto addr = [&](int y) -> unsigned { return col_base + (unsigned)(y / BY) * brow + (y % BY) * W; };
+  // Element j + r*TC of this lane's column: a lane offset plus a uniform
+  // offset per r (TC is a whole number of bricks for every schedule).
+  static_assert(TC % BY == 0, "column lanes span whole bricks");
+  const unsigned a0 = addr(j);
+  const unsigned rstride = (unsigned)(TC / BY) * brow;
+  auto elem = [&](const cx<T>* f, int r) -> gptr<cx<T>> { return Field<cx<T>>(const_cast<cx<T>*>(f)).plus((unsigned)r * rstride).at(a0); };
 
   const double dt = d.dt, dt2 = d.dt2, dt3 = d.dt3, dt6 = d.dt6;
   const bool col0 = (x == 0);
@@ -339,9 +351,9 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
   cx<T> sv[R];   // spectral input of the next stage
   double minj[2] = {0.0, 0.0};
 
-  if (stage == 0) {
+  if constexpr (stage == 0) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) sv[r] = Field<cx<T>>(d.wo)[addr(j + r * TC)];
+    for (int r = 0; r < R; ++r) sv[r] = *elem(d.wo, r);
     stage_twiddles<T, NT>(tw_y, tw_of(pool, NY), NY);
     if (tile == 0 && t == 0) {
       minj[0] = d.mean->M[0];
@@ -352,53 +364,94 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
     cx<T> pq[2][R], wv[R], av[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const unsigned a = addr(j + r * TC);
-      pq[0][r] = Field<cx<T>>(d.pm)[a];
-      pq[1][r] = Field<cx<T>>(d.qm)[a];
+      pq[0][r] = *elem(d.pm, r);
+      pq[1][r] = *elem(d.qm, r);
     }
-    // row-block partials for the mean ODE, summed per lane in a fixed order
-    double s0 = 0, s1 = 0, s2 = 0;
+    // Tile 0: row-block partials for the mean ODE (summed in a fixed order)
+    // and the mean state, parked in LDS until the ODE so that they occupy no
+    // registers across the transforms.
     if (mean_wave) {
+      double s0 = 0, s1 = 0, s2 = 0;
+      const gptr<double> pp = glob(d.partials);
       for (int i = t; i < d.row_tiles; i += RW) {
-        const gptr<double> pp = glob(d.partials);
         s0 += pp[(unsigned)i * 4 + 0];
         s1 += pp[(unsigned)i * 4 + 1];
         s2 += pp[(unsigned)i * 4 + 2];
       }
-    }
-    cx<T> pw[R];   // projection partner (Ny - y) of the kx = 0 column
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const unsigned a = addr(j + r * TC);
-      if (stage <= 3) wv[r] = Field<cx<T>>(d.wo)[a];
-      if (stage >= 2) av[r] = Field<cx<T>>(d.acc)[a];
-      if (col0) {
-        const int yp = (NY - (j + r * TC)) & (NY - 1);
-        pw[r] = Field<cx<T>>((stage == 1) ? d.wo : d.acc)[addr(yp)];
+      for (int off = RW / 2; off >= 1; off >>= 1) {
+        s0 += __shfl_xor(s0, off, RW);
+        s1 += __shfl_xor(s1, off, RW);
+        s2 += __shfl_xor(s2, off, RW);
+      }
+      if (t == 0) {
+        mean_in[0] = s0;
+        mean_in[1] = s1;
+        mean_in[2] = s2;
+        mean_in[3] = d.mean->M[0];
+        mean_in[4] = d.mean->M[1];
+        mean_in[5] = d.mean->Mo[0];
+        mean_in[6] = d.mean->Mo[1];
+        nstep_in = d.mean->nstep;
       }
     }
-    double Mcur[2] = {0, 0}, Mold[2] = {0, 0};
-    long long nstep = 0;
-    if (tile == 0 && t == 0) {
-      Mcur[0] = d.mean->M[0];
-      Mcur[1] = d.mean->M[1];
-      Mold[0] = d.mean->Mo[0];
-      Mold[1] = d.mean->Mo[1];
-      nstep = d.mean->nstep;
-    }
+    // av: the projection source (acc, or omega_old at stage 1, whose
+    // integrating factor is even in ky); wv: omega_old for stages 1-3.
+    auto load_av = [&]() {
+#pragma unroll
+      for (int r = 0; r < R; ++r) av[r] = *elem((stage == 1) ? d.wo : d.acc, r);
+    };
+    auto load_wv = [&]() {
+      if (stage == 2 || stage == 3) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) wv[r] = *elem(d.wo, r);
+      }
+    };
+    // Hermitian projection of the kx = 0 column (reality_func, .cu:1237-1242):
+    // a(y) <- (a(y) + conj a(Ny - y)) / 2 for y not 0 or Ny/2.  The column
+    // lives in tile 0's block, so partners are exchanged through LDS (free
+    // between transforms; the caller places a barrier before the next one).
+    auto project = [&]() {
+      if (stage == 1) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) wv[r] = av[r];
+      }
+      if (tile == 0) {
+        cx<T>* xcol = lds_mem;
+        __syncthreads();
+        if (c == 0) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) xcol[j + r * TC] = av[r];
+        }
+        __syncthreads();
+        if (col0) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int y = j + r * TC;
+            if ((y & (NY / 2 - 1)) != 0) av[r] = cscale((T)0.5, av[r] + cconj(xcol[(NY - y) & (NY - 1)]));
+          }
+        }
+      }
+    };
+    // Register pressure: with one forward transform at a time, wv is loaded
+    // and the projection done between the two so that P^, av and wv are not
+    // all live across a transform.
+    load_av();
+    if constexpr (GF == 2) load_wv();
     stage_twiddles<T, NT>(tw_y, tw_of(pool, NY), NY);
     __syncthreads();
     if constexpr (GF == 2) {
       fft_multi<NY, R, -1, 2>(pq, j, lds, tw_y);
+      project();
     } else {
-#pragma unroll
-      for (int f = 0; f < 2; ++f) {
-        cx<T>(&q)[1][R] = *reinterpret_cast<cx<T>(*)[1][R]>(&pq[f]);
-        if (f) __syncthreads();
-        fft_multi<NY, R, -1, 1>(q, j, lds, tw_y);
-      }
+      fft_multi<NY, R, -1, 1>(*reinterpret_cast<cx<T>(*)[1][R]>(&pq[0]), j, lds, tw_y);
+      project();
+      load_wv();
+      __syncthreads();
+      fft_multi<NY, R, -1, 1>(*reinterpret_cast<cx<T>(*)[1][R]>(&pq[1]), j, lds, tw_y);
     }
-    cx<T> an[R];
+    cx<T>* const dst = (stage == 4) ? d.wo : d.acc;
+    cx<T> a00 = {(T)0, (T)0};   // new acc (state at stage 4) of this lane's r = 0 mode
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int y = j + r * TC;
@@ -415,15 +468,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
         E2 = (T)e2d;
         E = (T)(e2d * e2d);
       }
-      const bool proj = col0 && (y & (NY / 2 - 1)) != 0;   // y not 0 or Ny/2
-      cx<T> a;
-      if (stage == 1) {
-        a = cscale(E, wv[r]);
-        if (proj) a = cscale((T)0.5, a + cconj(cscale(E, pw[r])));
-      } else {
-        a = av[r];
-        if (proj) a = cscale((T)0.5, a + cconj(pw[r]));
-      }
+      cx<T> a = (stage == 1) ? cscale(E, av[r]) : av[r];
       cx<T> s;
       if (stage == 1) {
         a = a + cscale((T)(dt6 * (double)E), nh);
@@ -438,50 +483,41 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
         a = a + cscale((T)dt6, nh);
         s = a;
       }
-      an[r] = a;
+      *elem(dst, r) = a;
+      if (r == 0) a00 = a;
       sv[r] = s;
     }
-    __syncthreads();   // every projection-partner read precedes the writes below
-    const Field<cx<T>> dst((stage == 4) ? d.wo : d.acc);
-#pragma unroll
-    for (int r = 0; r < R; ++r) dst.put(addr(j + r * TC), an[r]);
 
     // Mean-velocity ODE (host updates of .cu:729-821), FP64, tile 0 only.
-    if (mean_wave) {
-#pragma unroll
-      for (int off = RW / 2; off >= 1; off >>= 1) {
-        s0 += __shfl_xor(s0, off, RW);
-        s1 += __shfl_xor(s1, off, RW);
-        s2 += __shfl_xor(s2, off, RW);
+    if (tile == 0 && t == 0) {
+      MeanState* ms = d.mean;
+      const double s0 = mean_in[0];
+      const long long nstep = nstep_in;
+      const double g[2] = {-d.beta * mean_in[1], -d.beta * mean_in[2]};
+      double M[2] = {mean_in[3], mean_in[4]}, Mo[2] = {mean_in[5], mean_in[6]}, mm[2];
+      if (stage == 1) {
+        d.energy[nstep % d.ring_cap] = s0 * d.invN * 0.5;
+        Mo[0] = M[0];
+        Mo[1] = M[1];
+        if (d.mean_scheme == 1) { M[0] = d.e * Mo[0]; M[1] = d.e * Mo[1]; }
       }
-      if (t == 0) {
-        MeanState* ms = d.mean;
-        const double g[2] = {-d.beta * s1, -d.beta * s2};
-        double M[2] = {Mcur[0], Mcur[1]}, Mo[2] = {Mold[0], Mold[1]}, mm[2];
-        if (stage == 1) {
-          d.energy[nstep % d.ring_cap] = s0 * d.invN * 0.5;
-          Mo[0] = M[0];
-          Mo[1] = M[1];
-          if (d.mean_scheme == 1) { M[0] = d.e * Mo[0]; M[1] = d.e * Mo[1]; }
-        }
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          if (stage == 1) { M[i] += dt6 * d.e * g[i];  mm[i] = d.e2 * (Mo[i] + dt2 * g[i]); }
-          else if (stage == 2) { M[i] += dt3 * d.e2 * g[i]; mm[i] = d.e2 * (Mo[i] + dt2 * g[i]); }
-          else if (stage == 3) { M[i] += dt3 * d.e2 * g[i]; mm[i] = d.e * (Mo[i] + dt * g[i]); }
-          else { M[i] += dt6 * g[i]; mm[i] = M[i]; }
-        }
-        ms->M[0] = M[0]; ms->M[1] = M[1];
-        ms->Mo[0] = Mo[0]; ms->Mo[1] = Mo[1];
-        ms->m[0] = mm[0]; ms->m[1] = mm[1];
-        if (stage == 4) {
-          // check_func (.cu:884-897) on the new Re w^[0], owned by this lane
-          if (ms->nan_step < 0 && an[0].re != an[0].re) ms->nan_step = nstep + 1;
-          ms->nstep = nstep + 1;
-        }
-        minj[0] = mm[0];
-        minj[1] = mm[1];
+      for (int i = 0; i < 2; ++i) {
+        if (stage == 1) { M[i] += dt6 * d.e * g[i];  mm[i] = d.e2 * (Mo[i] + dt2 * g[i]); }
+        else if (stage == 2) { M[i] += dt3 * d.e2 * g[i]; mm[i] = d.e2 * (Mo[i] + dt2 * g[i]); }
+        else if (stage == 3) { M[i] += dt3 * d.e2 * g[i]; mm[i] = d.e * (Mo[i] + dt * g[i]); }
+        else { M[i] += dt6 * g[i]; mm[i] = M[i]; }
       }
+      ms->M[0] = M[0]; ms->M[1] = M[1];
+      ms->Mo[0] = Mo[0]; ms->Mo[1] = Mo[1];
+      ms->m[0] = mm[0]; ms->m[1] = mm[1];
+      if (stage == 4) {
+        // check_func (.cu:884-897) on the new Re w^[0], owned by this lane
+        if (ms->nan_step < 0 && a00.re != a00.re) ms->nan_step = nstep + 1;
+        ms->nstep = nstep + 1;
+      }
+      minj[0] = mm[0];
+      minj[1] = mm[1];
     }
   }
 
@@ -521,10 +557,9 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
     fft_multi<NY, R, +1, 3>(q3, j, lds, tw_y);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const unsigned a = addr(j + r * TC);
-      Field<cx<T>>(d.um).put(a, q3[0][r]);
-      Field<cx<T>>(d.vm).put(a, q3[1][r]);
-      Field<cx<T>>(d.wm).put(a, q3[2][r]);
+      *elem(d.um, r) = q3[0][r];
+      *elem(d.vm, r) = q3[1][r];
+      *elem(d.wm, r) = q3[2][r];
     }
   } else {
 #pragma unroll
@@ -533,9 +568,9 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, int stage, c
       make(q[0], f);
       __syncthreads();
       fft_multi<NY, R, +1, 1>(q, j, lds, tw_y);
-      const Field<cx<T>> dst((f == 0) ? d.um : (f == 1) ? d.vm : d.wm);
+      cx<T>* const dst = (f == 0) ? d.um : (f == 1) ? d.vm : d.wm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst.put(addr(j + r * TC), q[0][r]);
+      for (int r = 0; r < R; ++r) *elem(dst, r) = q[0][r];
     }
   }
 }
