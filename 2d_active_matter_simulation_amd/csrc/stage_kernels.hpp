@@ -109,6 +109,26 @@ __device__ __forceinline__ void stage_twiddles(cx<T>* dst, const cx<T>* __restri
   for (int i = threadIdx.x; i < n; i += NT) dst[i] = g[i];
 }
 
+// Workgroups are dispatched to the 8 XCDs (each with its own L2) round-robin
+// by workgroup id.  When K consecutive column tiles share 128-byte L2 lines
+// (bricks narrower than a line), tiles k*K .. k*K+K-1 are given to workgroup
+// ids that land on one XCD, so each line is fetched from HBM once instead of
+// once per XCD.  A permutation of [0, n): full groups of 8*K ids are
+// remapped, a trailing partial group is left as is.
+constexpr int kXcds = 8;
+template <int K>
+__device__ __forceinline__ int xcd_tile_order(int i, int n) {
+  if constexpr (K <= 1) {
+    return i;
+  } else {
+    constexpr int G = kXcds * K;
+    const int g = i / G;
+    if ((g + 1) * G > n) return i;
+    const int r = i - g * G;
+    return g * G + (r % kXcds) * K + r / kXcds;
+  }
+}
+
 // Deterministic block sum of NV doubles; result valid in thread 0.
 template <int NV, int NT>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* red /* LDS, NV * waves */) {
@@ -322,8 +342,11 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   __shared__ double mean_in[7];   // tile 0: sum|u|^2, sum Cx, sum Cy, M, M_old
   __shared__ long long nstep_in;
 
-  const int inst = blockIdx.x / tiles_per_inst;
-  const int tile = blockIdx.x - inst * tiles_per_inst;
+  // bricks of BY x W elements: how many column tiles share a 128-byte line
+  constexpr int kLine = 128 / (BY * W * (int)sizeof(cx<T>));
+  const int bid = xcd_tile_order<kLine>(blockIdx.x, gridDim.x);
+  const int inst = bid / tiles_per_inst;
+  const int tile = bid - inst * tiles_per_inst;
   const InstDesc<T>& d = descs[inst];
   const int t = threadIdx.x;
   const int c = t % W, j = t / W;
