@@ -90,26 +90,68 @@ struct Dft {
 constexpr int ilog(int n, int b) { return n < b ? 0 : 1 + ilog(n / b, b); }
 constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 
-// LDS index of element p of column c when W transforms are interleaved
-// (i = p*W + c), padded for the exchange written by a radix-R pass of stride
-// Ns: Ns*W extra elements every Ns*R*W elements, so that the groups of a
-// stride-Ns store land in disjoint bank ranges; no padding once a contiguous
-// run of Ns*W elements spans all 64 four-byte banks.  Conflict-free for the
-// store/load pairs of fft_multi (checked against a 32- and 64-bank model).
-// For a given exchange every lane's R stores and R loads sit at a lane base
-// plus offsets that do not depend on the lane, so fft_multi computes one base
-// per lane and uses the offsets as immediates.
+// LDS index of element i = p*W + c (position p of interleaved transform c)
+// for the exchange written by a radix-R pass of stride Ns.  The rule is chosen
+// against gfx950's LDS (64 banks of 4 bytes): the paired 8-byte accesses
+// fft_multi compiles to (ds_read2/ds_write2) are served 16 lanes at a time and
+// conflict-free when those lanes' 8-byte elements differ mod 16.
+//   8-byte elements:  P = min(Ns*W, 16/R) extra elements after every 16, none
+//                     once Ns*W >= 16 (stores already hit 16 consecutive
+//                     elements);
+//   16-byte elements: Ns*W extra per Ns*R*W, none once Ns*W*16 >= 256.
+// Every store and load pair of every schedule is conflict-free under that
+// rule, and for a given exchange each lane's R stores and R loads sit at one
+// lane base plus lane-independent offsets (lds_pad of the offset alone), so
+// fft_multi computes one base per lane and uses the offsets as immediates.
 template <int W, int R, int ESZ>
 __host__ __device__ constexpr int lds_pad(int i, int Ns) {
-  return (Ns * W * ESZ >= 256) ? i : i + (i / (Ns * R * W)) * (Ns * W);
+  if constexpr (ESZ == 8) {
+    if (Ns * W >= 16) return i;
+    const int P = (Ns * W < 16 / R) ? Ns * W : 16 / R;
+    return i + (i / 16) * P;
+  } else {
+    return (Ns * W * ESZ >= 256) ? i : i + (i / (Ns * R * W)) * (Ns * W);
+  }
 }
 // Elements of one LDS area for transforms of length L (covers every padding).
 template <int L, int W, int R> constexpr int lds_area_elems() { return L * W + (L * W) / R + 1; }
 
+// Twiddle table of fft_multi, in the order the radix passes read it: for
+// pass P >= 1 (stride Ns = R^P) entry off(P) + jm*(R-1) + (r-1) holds
+// w^(r*jm*L/(Ns*R)), jm < Ns, r in [1, R); for a trailing radix-RL pass entry
+// off(NP) + b*(RL-1) + (rr-1) holds w^(rr*b), b < L/RL, rr in [1, RL); with
+// w = exp(-2 pi i / L).  Lanes with consecutive jm (or b) read entries an
+// odd number of 8- or 16-byte elements apart, which never share a bank, and
+// each lane's R-1 twiddles of a pass sit at one base plus immediate offsets.
+// L - R entries (none when L == R).
+template <int L, int R>
+struct TwLayout {
+  static constexpr int NP = ilog(L, R);
+  static constexpr int RL = L / ipow(R, NP);
+  static constexpr int kSize = L - R;
+  static constexpr int kAlloc = kSize > 0 ? kSize : 1;
+  static constexpr int off(int P) { return ipow(R, P) - R; }
+  // exponent m of the entry i (w^m)
+  __host__ __device__ static int exponent(int i) {
+    int P = 1;
+    for (; P < NP; ++P)
+      if (i < off(P + 1)) {
+        const int k = i - off(P), jm = k / (R - 1), r = k % (R - 1) + 1;
+        return r * jm * (L / (ipow(R, P) * R));
+      }
+    if constexpr (RL > 1) {
+      const int k = i - off(NP), b = k / (RL - 1), rr = k % (RL - 1) + 1;
+      return rr * b;
+    } else {
+      return 0;   // no entries past the full passes
+    }
+  }
+};
+
 // NF length-L transforms done together (one LDS exchange per radix pass for
 // all of them).  lds points at this block's NF areas (lds_area_elems each) of
 // W interleaved transforms; this lane is transform lane j of column c.  tw
-// points at tw[m] = exp(-2 pi i m / L), m in [0, L) (normally an LDS copy).
+// points at the TwLayout<L, R> table (normally an LDS copy).
 // All lanes of the block must call this together (it contains block
 // barriers).  The first LDS access is a store after radix pass 1, so the
 // caller must place a barrier before the call if other lanes may still be
@@ -142,10 +184,10 @@ struct FftMulti {
         __syncthreads();
         load<Ns / R>(v, j, c, lds);
         const int jm = j & (Ns - 1);
-        const int tj = jm * (L / (Ns * R));
+        const cx<T>* twp = tw + (TwLayout<L, R>::off(P) - 1) + jm * (R - 1);
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          const cx<T> w = tw[r * tj];
+          const cx<T> w = twp[r];
 #pragma unroll
           for (int f = 0; f < NF; ++f) v[f][r] = DIR < 0 ? cmul(v[f][r], w) : cmulc(v[f][r], w);
         }
@@ -160,7 +202,7 @@ struct FftMulti {
 #pragma unroll
         for (int f = 0; f < NF; ++f)
 #pragma unroll
-          for (int r = 0; r < R; ++r) lds[f * AREA + a + r * Ns * W] = v[f][r];
+          for (int r = 0; r < R; ++r) lds[f * AREA + a + lds_pad<W, R, ESZ>(r * Ns * W, Ns)] = v[f][r];
       }
       pass<P + 1>(v, j, c, lds, tw);
     } else if constexpr (RL > 1) {
@@ -171,9 +213,10 @@ struct FftMulti {
 #pragma unroll
       for (int q = 0; q < R / RL; ++q) {
         const int b = j + q * S;
+        const cx<T>* twp = tw + (TwLayout<L, R>::off(NP) - 1) + b * (RL - 1);
         cx<T> w[RL];
 #pragma unroll
-        for (int rr = 1; rr < RL; ++rr) w[rr] = tw[rr * b];
+        for (int rr = 1; rr < RL; ++rr) w[rr] = twp[rr];
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
           cx<T> u[RL];

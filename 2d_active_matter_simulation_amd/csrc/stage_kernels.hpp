@@ -108,6 +108,13 @@ __device__ __forceinline__ void stage_twiddles(cx<T>* dst, const cx<T>* __restri
   const gptr<const cx<T>> g = glob(src);
   for (int i = threadIdx.x; i < n; i += NT) dst[i] = g[i];
 }
+// Builds the pass-ordered twiddle table of a length-L radix-R fft_multi in
+// LDS from the natural-order table src[m] = exp(-2 pi i m / L).
+template <class T, int NT, int L, int R>
+__device__ __forceinline__ void stage_fft_twiddles(cx<T>* dst, const cx<T>* __restrict__ src) {
+  const gptr<const cx<T>> g = glob(src);
+  for (int i = threadIdx.x; i < TwLayout<L, R>::kSize; i += NT) dst[i] = g[TwLayout<L, R>::exponent(i)];
+}
 
 // Workgroups are dispatched to the 8 XCDs (each with its own L2) round-robin
 // by workgroup id.  When K consecutive column tiles share 128-byte L2 lines
@@ -168,7 +175,7 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   static_assert(G == 1 || G == 3, "group of 1 or 3 fields");
   constexpr int GR = G == 3 ? 2 : 1;   // R2C group
   __shared__ cx<T> lds_mem[G * lds_area_elems<NX2, RB, R>()];
-  __shared__ cx<T> tw_h[NX2];          // exp(-2 pi i m / (Nx/2)), the half-length transforms
+  __shared__ cx<T> tw_h[TwLayout<NX2, R>::kAlloc];   // the half-length transforms
   __shared__ cx<T> tw_n[NX2];          // exp(-2 pi i k / Nx), k < Nx/2, the even/odd split
   __shared__ double red[4 * ((NT + 63) / 64)];
 
@@ -196,7 +203,7 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     f3[1][r] = gvm[a];
     f3[2][r] = gwm[a];
   }
-  stage_twiddles<T, NT>(tw_h, tw_of(pool, NX2), NX2);
+  stage_fft_twiddles<T, NT, NX2, R>(tw_h, tw_of(pool, NX2));
   stage_twiddles<T, NT>(tw_n, tw_of(pool, 2 * NX2), NX2);
 
   // Row C2R of a half spectrum X[0..NX2) (X[NX2] = 0, Im X[0] ignored):
@@ -338,7 +345,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   constexpr int GMAX = GF > GI ? GF : GI;
   static_assert((GF == 1 || GF == 2) && (GI == 1 || GI == 3), "field groups");
   __shared__ cx<T> lds_mem[GMAX * lds_area_elems<NY, W, R>()];
-  __shared__ cx<T> tw_y[NY];
+  __shared__ cx<T> tw_y[TwLayout<NY, R>::kAlloc];
   __shared__ double mean_in[7];   // tile 0: sum|u|^2, sum Cx, sum Cy, M, M_old
   __shared__ long long nstep_in;
 
@@ -377,7 +384,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   if constexpr (stage == 0) {
 #pragma unroll
     for (int r = 0; r < R; ++r) sv[r] = *elem(d.wo, r);
-    stage_twiddles<T, NT>(tw_y, tw_of(pool, NY), NY);
+    stage_fft_twiddles<T, NT, NY, R>(tw_y, tw_of(pool, NY));
     if (tile == 0 && t == 0) {
       minj[0] = d.mean->M[0];
       minj[1] = d.mean->M[1];
@@ -461,7 +468,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     // all live across a transform.
     load_av();
     if constexpr (GF == 2) load_wv();
-    stage_twiddles<T, NT>(tw_y, tw_of(pool, NY), NY);
+    stage_fft_twiddles<T, NT, NY, R>(tw_y, tw_of(pool, NY));
     __syncthreads();
     if constexpr (GF == 2) {
       fft_multi<NY, R, -1, 2>(pq, j, lds, tw_y);
