@@ -1,14 +1,23 @@
 // Instantiation helpers shared by kernels_f32.hip and kernels_f64.hip.
 #pragma once
+#include <hip/hip_ext.h>
+
 #include "registry.hpp"
 #include "stage_kernels.hpp"
 
 namespace af {
 
+// Launches through hipExtLaunchKernelGGL, which also records the profiling
+// events of g_launch_events when they are set.
+template <class K, class... A>
+inline void launch(K kernel, dim3 grid, dim3 block, hipStream_t s, A... args) {
+  hipExtLaunchKernelGGL(kernel, grid, block, 0, s, g_launch_events.start, g_launch_events.stop, 0, args...);
+}
+
 template <class T, int NX2, int RB, int R, int BX, int G>
 hipError_t launch_row(const void* descs, int ninst, int tiles, const void* pool, hipStream_t s) {
-  hipLaunchKernelGGL((row_pass<T, NX2, RB, R, BX, G>), dim3(ninst * tiles), dim3(RB * (NX2 / R)), 0, s,
-                     (const InstDesc<T>*)descs, tiles, (const cx<T>*)pool);
+  launch(row_pass<T, NX2, RB, R, BX, G>, dim3(ninst * tiles), dim3(RB * (NX2 / R)), s, (const InstDesc<T>*)descs,
+         tiles, (const cx<T>*)pool);
   return hipGetLastError();
 }
 
@@ -18,11 +27,11 @@ hipError_t launch_col(const void* descs, int ninst, int tiles, int stage, const 
   const InstDesc<T>* dd = (const InstDesc<T>*)descs;
   const cx<T>* pp = (const cx<T>*)pool;
   switch (stage) {
-    case 0: hipLaunchKernelGGL((col_pass<T, NY, W, R, BY, GF, GI, 0>), grid, block, 0, s, dd, tiles, pp); break;
-    case 1: hipLaunchKernelGGL((col_pass<T, NY, W, R, BY, GF, GI, 1>), grid, block, 0, s, dd, tiles, pp); break;
-    case 2: hipLaunchKernelGGL((col_pass<T, NY, W, R, BY, GF, GI, 2>), grid, block, 0, s, dd, tiles, pp); break;
-    case 3: hipLaunchKernelGGL((col_pass<T, NY, W, R, BY, GF, GI, 3>), grid, block, 0, s, dd, tiles, pp); break;
-    case 4: hipLaunchKernelGGL((col_pass<T, NY, W, R, BY, GF, GI, 4>), grid, block, 0, s, dd, tiles, pp); break;
+    case 0: launch(col_pass<T, NY, W, R, BY, GF, GI, 0>, grid, block, s, dd, tiles, pp); break;
+    case 1: launch(col_pass<T, NY, W, R, BY, GF, GI, 1>, grid, block, s, dd, tiles, pp); break;
+    case 2: launch(col_pass<T, NY, W, R, BY, GF, GI, 2>, grid, block, s, dd, tiles, pp); break;
+    case 3: launch(col_pass<T, NY, W, R, BY, GF, GI, 3>, grid, block, s, dd, tiles, pp); break;
+    case 4: launch(col_pass<T, NY, W, R, BY, GF, GI, 4>, grid, block, s, dd, tiles, pp); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

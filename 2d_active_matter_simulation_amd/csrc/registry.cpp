@@ -14,6 +14,9 @@
 
 namespace af {
 
+thread_local LaunchEvents g_launch_events;
+
+
 const RowCfg* row_cfgs_f32(int* n);
 const ColCfg* col_cfgs_f32(int* n);
 const Schedule* schedules_f32(int* n);

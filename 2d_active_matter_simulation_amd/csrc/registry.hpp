@@ -36,6 +36,12 @@ const Schedule* active_schedule(int precision, int nx2, int ny);
 bool select_schedule(int precision, int nx2, int ny, int index);
 const RowCfg* find_row_cfg(int precision, const Schedule* s);
 const ColCfg* find_col_cfg(int precision, const Schedule* s);
+// When set (profiling), the next row/column launch on this thread records
+// these events at the kernel's own start and end (hipExtLaunchKernel).
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local LaunchEvents g_launch_events;
 ConvLaunch to_bricks_launch(int precision);
 ConvLaunch from_bricks_launch(int precision);
 

@@ -404,35 +404,46 @@ af_status af_run_use_graph(af_run* r, int on) {
 af_status af_run_profile(af_run* r, int nsteps, double* row_ms, double* col_ms, double* total_ms) {
   if (!r || nsteps <= 0) return fail(AF_EINVAL, "bad arguments");
   if (!r->ready) return fail(AF_EINVAL, "state not set");
-  const int nev = 8 * nsteps + 1;
-  std::vector<hipEvent_t> ev(nev, nullptr);
+  // events at each kernel's own start and end, plus two around the whole run
+  const int nl = 8 * nsteps;
+  std::vector<hipEvent_t> ev(2 * nl + 2, nullptr);
   for (auto& x : ev)
-    if (hipEventCreate(&x) != hipSuccess) return fail(AF_EDEVICE, "event create");
+    if (hipEventCreate(&x) != hipSuccess) {
+      for (auto& y : ev)
+        if (y) (void)hipEventDestroy(y);
+      return fail(AF_EDEVICE, "event create");
+    }
   const void* desc = r->base + r->g.off_desc;
   const void* pool = r->base + r->g.off_pool;
-  hipError_t e = hipEventRecord(ev[0], r->stream);
-  int k = 1;
+  hipError_t e = hipEventRecord(ev[2 * nl], r->stream);
+  int k = 0;
   for (int s = 0; s < nsteps && e == hipSuccess; ++s)
     for (int stage = 1; stage <= 4 && e == hipSuccess; ++stage) {
+      g_launch_events = {ev[2 * k], ev[2 * k + 1]};
+      ++k;
       e = r->g.rc->launch(desc, 1, r->g.row_tiles, pool, r->stream);
-      if (e == hipSuccess) e = hipEventRecord(ev[k++], r->stream);
+      g_launch_events = {ev[2 * k], ev[2 * k + 1]};
+      ++k;
       if (e == hipSuccess) e = r->g.cc->launch(desc, 1, r->g.col_tiles, stage, pool, r->stream);
-      if (e == hipSuccess) e = hipEventRecord(ev[k++], r->stream);
     }
+  g_launch_events = {};
+  if (e == hipSuccess) e = hipEventRecord(ev[2 * nl + 1], r->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
   double rs = 0, cs = 0;
   if (e == hipSuccess) {
-    for (int i = 1; i < nev; ++i) {
+    for (int i = 0; i < nl && e == hipSuccess; ++i) {
       float ms = 0;
-      (void)hipEventElapsedTime(&ms, ev[i - 1], ev[i]);
-      if (i % 2 == 1) rs += ms; else cs += ms;
+      e = hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]);
+      if (i % 2 == 0) rs += ms; else cs += ms;
     }
     float tot = 0;
-    (void)hipEventElapsedTime(&tot, ev[0], ev[nev - 1]);
-    if (row_ms) *row_ms = rs / (4.0 * nsteps);
-    if (col_ms) *col_ms = cs / (4.0 * nsteps);
-    if (total_ms) *total_ms = tot;
-    r->steps += nsteps;
+    if (e == hipSuccess) e = hipEventElapsedTime(&tot, ev[2 * nl], ev[2 * nl + 1]);
+    if (e == hipSuccess) {
+      if (row_ms) *row_ms = rs / (4.0 * nsteps);
+      if (col_ms) *col_ms = cs / (4.0 * nsteps);
+      if (total_ms) *total_ms = tot;
+      r->steps += nsteps;
+    }
   }
   for (auto& x : ev) (void)hipEventDestroy(x);
   if (e != hipSuccess) return hip_fail(e, "profile");

@@ -93,11 +93,11 @@ long long af_run_steps_done(const af_run* r);
 af_status af_run_diagnostics(af_run* r, double* energy, double* enstrophy, double* spectrum, int nshell);
 void af_run_destroy(af_run* r);
 
-/* Per-kernel timing with HIP events on the run's stream: runs nsteps steps
- * with an event pair around every launch and returns the mean duration in
- * milliseconds of the row pass and of the column pass (stages 1-4), and the
- * wall time of the whole bracketed region.  No reference counterpart
- * (measurement hook for bench.py). */
+/* Per-kernel timing with HIP events on the run's stream: runs nsteps steps,
+ * each launch recording an event pair at its kernel's own start and end
+ * (hipExtLaunchKernel), and returns the mean duration in milliseconds of the
+ * row pass and of the column pass (stages 1-4), and the wall time of the whole
+ * region.  No reference counterpart (measurement hook for bench.py). */
 af_status af_run_profile(af_run* r, int nsteps, double* row_ms, double* col_ms, double* total_ms);
 /* Use (1) or bypass (0) a captured hipGraph for one step (default 1). */
 af_status af_run_use_graph(af_run* r, int on);
