@@ -37,9 +37,9 @@
 
 namespace af {
 
-// Pointers read from an instance descriptor are generic; the fields live in
-// device global memory, so accesses go through an address-space-1 view (global
-// instructions with a scalar base and a 32-bit lane offset instead of FLAT).
+// Pointers read from an instance descriptor are generic; the small arrays
+// (partials, twiddle pool) are read through an address-space-1 view (global
+// instructions instead of FLAT).  Fields go through BufField below.
 #if defined(__HIP_DEVICE_COMPILE__)
 template <class T> using gptr = __attribute__((address_space(1))) T*;
 #else   // host pass of hipcc: these functions are never executed on the host
@@ -47,23 +47,43 @@ template <class T> using gptr = T*;
 #endif
 template <class T> __device__ __forceinline__ gptr<T> glob(T* p) { return (gptr<T>)p; }
 
-// Element idx of a global array as a 32-bit byte offset from the (uniform)
-// base, the form the scalar-base addressing of global loads takes.
+// Buffer-resource view of a complex field: the base lives in a 4-SGPR buffer
+// descriptor, a lane's byte offset in a VGPR and a uniform byte offset (e.g. a
+// row group's) in an SGPR, so an access costs no per-lane address arithmetic.
+// Raw buffer accesses (stride 0, descriptor word 3 of the gfx9 family);
+// offsets stay far below num_records (fields hold at most 2^27 bytes).
 template <class T>
-struct Field {
-  gptr<char> b;
-  __device__ __forceinline__ explicit Field(T* p) : b((gptr<char>)glob(p)) {}
-  __device__ __forceinline__ Field(const Field&) = default;
-  __device__ __forceinline__ gptr<T> at(unsigned idx) const { return (gptr<T>)(b + idx * (unsigned)sizeof(T)); }
-  __device__ __forceinline__ T operator[](unsigned idx) const { return *at(idx); }
-  __device__ __forceinline__ void put(unsigned idx, T v) const { *at(idx) = v; }
-  // the same array viewed from a uniform element offset further on (keeps
-  // per-row-group offsets in the scalar base instead of one VGPR per group)
-  __device__ __forceinline__ Field plus(unsigned u) const {
-    Field f(*this);
-    f.b = b + (size_t)u * sizeof(T);
-    return f;
+struct BufField {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // 16-byte elements (complex double) are stored with global_store_dwordx4:
+  // the compiler treats a buffer_store_dwordx4 with an SGPR soffset as free of
+  // the store-data hazard, and on gfx950 its data registers were observed to be
+  // overwritten before the store read them (corrupt dwords in FP64 fields under
+  // register pressure).  8-byte stores and all loads use the buffer path.
+  __amdgpu_buffer_rsrc_t rs;
+  gptr<char> base;
+  __device__ __forceinline__ explicit BufField(const T* p)
+      : rs(__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(p), 0, 0x7fffffff, 0x00020000)),
+        base((gptr<char>)glob(const_cast<T*>(p))) {}
+  __device__ __forceinline__ T ld(unsigned vbyte, unsigned sbyte) const {
+    static_assert(sizeof(T) == 8 || sizeof(T) == 16, "complex float or double");
+    if constexpr (sizeof(T) == 8)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, vbyte, sbyte, 0));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(rs, vbyte, sbyte, 0));
   }
+  __device__ __forceinline__ void st(unsigned vbyte, unsigned sbyte, T v) const {
+    if constexpr (sizeof(T) == 8)
+      __builtin_amdgcn_raw_buffer_store_b64(
+          __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(rs, 0, 0, 0)), v), rs, vbyte, sbyte, 0);
+    else   // 16-byte stores go through a global store (see above)
+      *(gptr<T>)(base + sbyte + vbyte) = v;
+  }
+#else   // host pass of hipcc: never executed on the host
+  __device__ explicit BufField(const T*) {}
+  __device__ T ld(unsigned, unsigned) const { return T{}; }
+  __device__ void st(unsigned, unsigned, T) const {}
+#endif
 };
 
 // 1/x: hardware reciprocal (1 ulp) in FP32, correctly rounded division in FP64.
@@ -194,14 +214,20 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   const unsigned tile_base = (unsigned)tile * (NX2 / BX) * (RB * BX) + yy * BX;
   auto addr = [&](int k) -> unsigned { return tile_base + (unsigned)(k / BX) * (RB * BX) + (k % BX); };
 
-  const Field<cx<T>> gum(d.um), gvm(d.vm), gwm(d.wm);
+  // element j + r*TR of this lane's row: lane offset addr(j) plus r*TR*RB
+  // (TR is a whole number of bricks)
+  constexpr unsigned ESZ = sizeof(cx<T>);
+  const unsigned vb = addr(j) * ESZ;
+  auto rb = [](int r) -> unsigned { return (unsigned)r * TR * RB * ESZ; };
   cx<T> f3[3][R];
+  {
+    const BufField<cx<T>> gum(d.um), gvm(d.vm), gwm(d.wm);
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const unsigned a = addr(j + r * TR);
-    f3[0][r] = gum[a];
-    f3[1][r] = gvm[a];
-    f3[2][r] = gwm[a];
+    for (int r = 0; r < R; ++r) {
+      f3[0][r] = gum.ld(vb, rb(r));
+      f3[1][r] = gvm.ld(vb, rb(r));
+      f3[2][r] = gwm.ld(vb, rb(r));
+    }
   }
   stage_fft_twiddles<T, NT, NX2, R>(tw_h, tw_of(pool, NX2));
   stage_twiddles<T, NT>(tw_n, tw_of(pool, 2 * NX2), NX2);
@@ -311,13 +337,13 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   }
 
   const T tkx = (T)d.tkx;
+  const BufField<cx<T>> gpm(d.pm), gqm(d.qm);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int k = j + r * TR;
     const T kx = tkx * (T)k;
-    const unsigned a = addr(k);
-    Field<cx<T>>(d.pm).put(a, cx<T>{kx * f2[0][r].im, -kx * f2[0][r].re});   // -i kx X1
-    Field<cx<T>>(d.qm).put(a, f2[1][r]);
+    gpm.st(vb, rb(r), cx<T>{kx * f2[0][r].im, -kx * f2[0][r].re});   // -i kx X1
+    gqm.st(vb, rb(r), f2[1][r]);
   }
 
   double sums[3] = {esum, gxs, gys};
@@ -367,9 +393,11 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   // Element j + r*TC of this lane's column: a lane offset plus a uniform
   // offset per r (TC is a whole number of bricks for every schedule).
   static_assert(TC % BY == 0, "column lanes span whole bricks");
-  const unsigned a0 = addr(j);
-  const unsigned rstride = (unsigned)(TC / BY) * brow;
-  auto elem = [&](const cx<T>* f, int r) -> gptr<cx<T>> { return Field<cx<T>>(const_cast<cx<T>*>(f)).plus((unsigned)r * rstride).at(a0); };
+  constexpr unsigned ESZ = sizeof(cx<T>);
+  const unsigned vb = addr(j) * ESZ;
+  const unsigned rstride = (unsigned)(TC / BY) * brow * ESZ;   // bytes, uniform
+  auto ld = [&](const cx<T>* f, int r) -> cx<T> { return BufField<cx<T>>(f).ld(vb, (unsigned)r * rstride); };
+  auto st = [&](cx<T>* f, int r, cx<T> v) { BufField<cx<T>>(f).st(vb, (unsigned)r * rstride, v); };
 
   const double dt = d.dt, dt2 = d.dt2, dt3 = d.dt3, dt6 = d.dt6;
   const bool col0 = (x == 0);
@@ -383,7 +411,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 
   if constexpr (stage == 0) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) sv[r] = *elem(d.wo, r);
+    for (int r = 0; r < R; ++r) sv[r] = ld(d.wo, r);
     stage_fft_twiddles<T, NT, NY, R>(tw_y, tw_of(pool, NY));
     if (tile == 0 && t == 0) {
       minj[0] = d.mean->M[0];
@@ -394,8 +422,8 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     cx<T> pq[2][R], wv[R], av[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      pq[0][r] = *elem(d.pm, r);
-      pq[1][r] = *elem(d.qm, r);
+      pq[0][r] = ld(d.pm, r);
+      pq[1][r] = ld(d.qm, r);
     }
     // Tile 0: row-block partials for the mean ODE (summed in a fixed order)
     // and the mean state, parked in LDS until the ODE so that they occupy no
@@ -429,12 +457,12 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     // integrating factor is even in ky); wv: omega_old for stages 1-3.
     auto load_av = [&]() {
 #pragma unroll
-      for (int r = 0; r < R; ++r) av[r] = *elem((stage == 1) ? d.wo : d.acc, r);
+      for (int r = 0; r < R; ++r) av[r] = ld((stage == 1) ? d.wo : d.acc, r);
     };
     auto load_wv = [&]() {
       if (stage == 2 || stage == 3) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) wv[r] = *elem(d.wo, r);
+        for (int r = 0; r < R; ++r) wv[r] = ld(d.wo, r);
       }
     };
     // Hermitian projection of the kx = 0 column (reality_func, .cu:1237-1242):
@@ -513,7 +541,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
         a = a + cscale((T)dt6, nh);
         s = a;
       }
-      *elem(dst, r) = a;
+      st(dst, r, a);
       if (r == 0) a00 = a;
       sv[r] = s;
     }
@@ -587,9 +615,9 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     fft_multi<NY, R, +1, 3>(q3, j, lds, tw_y);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      *elem(d.um, r) = q3[0][r];
-      *elem(d.vm, r) = q3[1][r];
-      *elem(d.wm, r) = q3[2][r];
+      st(d.um, r, q3[0][r]);
+      st(d.vm, r, q3[1][r]);
+      st(d.wm, r, q3[2][r]);
     }
   } else {
 #pragma unroll
@@ -600,7 +628,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
       fft_multi<NY, R, +1, 1>(q, j, lds, tw_y);
       cx<T>* const dst = (f == 0) ? d.um : (f == 1) ? d.vm : d.wm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) *elem(dst, r) = q[0][r];
+      for (int r = 0; r < R; ++r) st(dst, r, q[0][r]);
     }
   }
 }
