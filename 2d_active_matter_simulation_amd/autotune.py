@@ -67,7 +67,7 @@ def main():
     ap.add_argument("--out", default=os.path.join(HERE, "schedules.txt"))
     ap.add_argument("--dry", action="store_true", help="print, do not write the table")
     args = ap.parse_args()
-    lines = ["# prec nx ny by bx row_radix row_group col_radix col_fwd_group col_inv_group  (ms/step)"]
+    lines = ["# prec nx ny by bx row_radix row_group col_radix col_fwd_group col_inv_group col_width  (ms/step)"]
     for prec in args.precisions:
         for n in args.sizes:
             scheds = af.schedules(prec, n)

@@ -2,8 +2,10 @@
 """Generates kernels_f32.hip / kernels_f64.hip: the instantiated row/column
 pass kernels and the schedule table of each (precision, Nx/2, Ny).
 
-A schedule fixes the brick shape (BY rows x BX columns) and, for each pass,
-the radix R and how many fields share an LDS exchange.  For every size there
+A schedule fixes the brick shape (BY rows x BX columns), for each pass the
+radix R and how many fields share an LDS exchange, and the number of columns
+per column-pass block (BX or a divisor of it: narrower blocks mean more of
+them, which lets a lone run overlap one block's loads with another's math).  For every size there
 is one default schedule (used unless a tuned table in schedules.txt says
 otherwise) and, for square grids from 256^2 to 4096^2, a few alternatives for
 the autotuner (autotune.py).  Constraints checked here: at most 512 lanes per
@@ -31,7 +33,7 @@ def lds_col(ny, W, R, GF, GI, esz):
 
 
 def default_schedule(prec, nx, ny):
-    """(by, bx, row_r, row_g, col_r, col_gf, col_gi)"""
+    """(by, bx, row_r, row_g, col_r, col_gf, col_gi, col_w)"""
     nx2 = nx // 2
     if prec == 32:
         by = 2 if nx2 >= 2048 else 4
@@ -43,7 +45,7 @@ def default_schedule(prec, nx, ny):
         bx = 1 if ny >= 4096 else 2
         row = (4, 3) if nx2 == 16 else ((8, 1) if nx2 >= 1024 else (8, 3))
         col = (8, 1, 1) if ny >= 2048 else (8, 2, 3)
-    return (by, bx) + row + col
+    return (by, bx) + row + col + (bx,)
 
 
 def candidates(prec, n):
@@ -53,9 +55,13 @@ def candidates(prec, n):
     if prec == 32:
         alts = [(4, 2, 4, 3, 4, 2, 3), (4, 4, 8, 3, 8, 2, 3), (2, 4, 8, 3, 8, 2, 3), (4, 2, 8, 1, 8, 1, 1),
                 (4, 2, 8, 3, 16, 1, 1), (2, 2, 8, 3, 8, 2, 3), (4, 2, 4, 3, 8, 2, 3), (4, 2, 8, 3, 4, 2, 3)]
+        narrow = [(4, 2, 4, 3, 4, 2, 3, 1), (4, 2, 8, 1, 8, 1, 1, 1), (4, 2, 8, 3, 8, 2, 3, 1),
+                  (4, 4, 8, 3, 8, 2, 3, 2), (2, 2, 8, 1, 16, 1, 1, 1), (4, 2, 4, 3, 8, 2, 3, 1)]
     else:
         alts = [(4, 2, 4, 3, 4, 2, 3), (4, 1, 8, 3, 8, 2, 3), (2, 2, 8, 3, 8, 2, 3), (4, 2, 8, 1, 8, 1, 1),
                 (4, 2, 4, 1, 4, 1, 1), (2, 2, 4, 3, 8, 1, 1)]
+        narrow = [(4, 2, 4, 3, 4, 2, 3, 1), (4, 2, 8, 3, 8, 2, 3, 1), (4, 2, 8, 1, 8, 1, 1, 1)]
+    alts = [a + (a[1],) for a in alts] + narrow
     for s in alts:
         if s not in out and feasible(prec, n, n, s):
             out.append(s)
@@ -63,17 +69,17 @@ def candidates(prec, n):
 
 
 def feasible(prec, nx, ny, s):
-    by, bx, rr, rg, cr, gf, gi = s
+    by, bx, rr, rg, cr, gf, gi, cw = s
     esz = 8 if prec == 32 else 16
     nx2 = nx // 2
     if rr > nx2 or cr > ny:
         return False
     tr, tc = nx2 // rr, ny // cr
-    if tr % bx or nx2 % bx or ny % by:
+    if tr % bx or nx2 % bx or ny % by or bx % cw or tc % by:
         return False
-    if by * tr > 512 or bx * tc > 512:
+    if by * tr > 512 or cw * tc > 512:
         return False
-    if lds_row(nx2, by, rr, rg, esz) > LDS_MAX or lds_col(ny, bx, cr, gf, gi, esz) > LDS_MAX:
+    if lds_row(nx2, by, rr, rg, esz) > LDS_MAX or lds_col(ny, cw, cr, gf, gi, esz) > LDS_MAX:
         return False
     return True
 
@@ -81,8 +87,8 @@ def feasible(prec, nx, ny, s):
 HDR = """// {what} instantiations of the stage kernels and the schedule table.
 // Generated by gen_kernel_tables.py -- edit that script, not this file.
 // Row kernel:    (Nx/2, RB = brick height, R, BX = brick width, G)
-// Column kernel: (Ny, W = brick width, R, BY = brick height, GF, GI)
-// Schedule:      (Nx/2, Ny, BY, BX, row R, row G, col R, col GF, col GI, default)
+// Column kernel: (Ny, W = columns per block, R, BY, BX = brick height/width, GF, GI)
+// Schedule:      (Nx/2, Ny, BY, BX, row R, row G, col R, col GF, col GI, col W, default)
 #include "kernels_inst.hpp"
 
 namespace af {{
@@ -122,14 +128,14 @@ def gen(prec):
                 for s in candidates(prec, nx)[1:]:
                     scheds.append((nx // 2, ny) + s + (0,))
     rows, cols = set(), set()
-    for (nx2, ny, by, bx, rr, rg, cr, gf, gi, _) in scheds:
+    for (nx2, ny, by, bx, rr, rg, cr, gf, gi, cw, _) in scheds:
         rows.add((nx2, by, rr, bx, rg))
-        cols.add((ny, bx, cr, by, gf, gi))
+        cols.add((ny, cw, cr, by, bx, gf, gi))
     rl = [f"    {{{a}, {b}, {r}, {c}, {b * (a // r)}, launch_row<T, {a}, {b}, {r}, {c}, {g}>, {g}}},"
           for (a, b, r, c, g) in sorted(rows)]
-    cl = [f"    {{{a}, {w}, {r}, {by}, {w * (a // r)}, launch_col<T, {a}, {w}, {r}, {by}, {gf}, {gi}>, {gf}, {gi}}},"
-          for (a, w, r, by, gf, gi) in sorted(cols)]
-    sl = [f"    {{{a}, {b}, {c}, {d}, {e}, {f}, {g}, {h}, {i}, {j}}}," for (a, b, c, d, e, f, g, h, i, j) in scheds]
+    cl = [f"    {{{a}, {w}, {r}, {by}, {bx}, {w * (a // r)}, launch_col<T, {a}, {w}, {r}, {by}, {bx}, {gf}, {gi}>, "
+          f"{gf}, {gi}}}," for (a, w, r, by, bx, gf, gi) in sorted(cols)]
+    sl = ["    {" + ", ".join(map(str, t)) + "}," for t in scheds]
     text = HDR.format(what=f"FP{prec}", T=T, tag=tag, rows="\n".join(rl), cols="\n".join(cl), scheds="\n".join(sl))
     with open(os.path.join(HERE, f"kernels_{tag}.hip"), "w") as f:
         f.write(text)

@@ -21,17 +21,17 @@ hipError_t launch_row(const void* descs, int ninst, int tiles, const void* pool,
   return hipGetLastError();
 }
 
-template <class T, int NY, int W, int R, int BY, int GF, int GI>
+template <class T, int NY, int W, int R, int BY, int BX, int GF, int GI>
 hipError_t launch_col(const void* descs, int ninst, int tiles, int stage, const void* pool, hipStream_t s) {
   const dim3 grid(ninst * tiles), block(W * (NY / R));
   const InstDesc<T>* dd = (const InstDesc<T>*)descs;
   const cx<T>* pp = (const cx<T>*)pool;
   switch (stage) {
-    case 0: launch(col_pass<T, NY, W, R, BY, GF, GI, 0>, grid, block, s, dd, tiles, pp); break;
-    case 1: launch(col_pass<T, NY, W, R, BY, GF, GI, 1>, grid, block, s, dd, tiles, pp); break;
-    case 2: launch(col_pass<T, NY, W, R, BY, GF, GI, 2>, grid, block, s, dd, tiles, pp); break;
-    case 3: launch(col_pass<T, NY, W, R, BY, GF, GI, 3>, grid, block, s, dd, tiles, pp); break;
-    case 4: launch(col_pass<T, NY, W, R, BY, GF, GI, 4>, grid, block, s, dd, tiles, pp); break;
+    case 0: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, 0>, grid, block, s, dd, tiles, pp); break;
+    case 1: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, 1>, grid, block, s, dd, tiles, pp); break;
+    case 2: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, 2>, grid, block, s, dd, tiles, pp); break;
+    case 3: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, 3>, grid, block, s, dd, tiles, pp); break;
+    case 4: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, 4>, grid, block, s, dd, tiles, pp); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

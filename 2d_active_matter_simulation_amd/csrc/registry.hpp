@@ -19,12 +19,12 @@ struct RowCfg {
   int g;
 };
 struct ColCfg {
-  int ny, w, r, by, threads;
+  int ny, w, r, by, bx, threads;   // w: columns per block (divides the brick width bx)
   ColLaunch launch;
   int gf, gi;
 };
 struct Schedule {
-  int nx2, ny, by, bx, row_r, row_g, col_r, col_gf, col_gi, is_default;
+  int nx2, ny, by, bx, row_r, row_g, col_r, col_gf, col_gi, col_w, is_default;
 };
 
 // Schedules instantiated for (precision, nx2, ny); index 0 is the default.

@@ -357,13 +357,14 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 }
 
 // ---------------------------------------------------------------------------
-// Column pass.  Block = W adjacent columns x TC = NY/R lanes per column.
+// Column pass.  Block = W adjacent columns (of one BX-wide brick column) x
+// TC = NY/R lanes per column.
 // stage 0: initial pass only (v^ from the state, inverse transforms).
 // stage 1..4: full stage.  The stage is a template argument: each stage's
 // kernel keeps only the state it reads live (fewer registers, more waves).  GF = 2 / GI = 3: the forward (P, Q) and inverse
 // (v_x, v_y, omega) transforms share LDS exchanges; 1: one at a time.
 // ---------------------------------------------------------------------------
-template <class T, int NY, int W, int R, int BY, int GF, int GI, int stage>
+template <class T, int NY, int W, int R, int BY, int BX, int GF, int GI, int stage>
 __global__ void __launch_bounds__(W * (NY / R))
 col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>* __restrict__ pool) {
   constexpr int TC = NY / R;
@@ -375,7 +376,8 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   __shared__ double mean_in[7];   // tile 0: sum|u|^2, sum Cx, sum Cy, M, M_old
   __shared__ long long nstep_in;
 
-  // bricks of BY x W elements: how many column tiles share a 128-byte line
+  // a tile reads BY x W elements of each BY-row band: how many column tiles
+  // share a 128-byte line
   constexpr int kLine = 128 / (BY * W * (int)sizeof(cx<T>));
   const int bid = xcd_tile_order<kLine>(blockIdx.x, gridDim.x);
   const int inst = bid / tiles_per_inst;
@@ -387,9 +389,11 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   const int nx2 = d.nx >> 1;
   LdsView<T, NY, W, R> lds{lds_mem, c};
 
-  const unsigned col_base = (unsigned)tile * (BY * W) + c;
-  const unsigned brow = (unsigned)(nx2 / W) * (BY * W);  // elements per brick row
-  auto addr = [&](int y) -> unsigned { return col_base + (unsigned)(y / BY) * brow + (y % BY) * W; };
+  // bricks of BY x BX; this block's W columns sit inside one brick column
+  static_assert(BX % W == 0, "column block within one brick column");
+  const unsigned col_base = (unsigned)(tile * W / BX) * (BY * BX) + (unsigned)(tile * W % BX) + c;
+  const unsigned brow = (unsigned)nx2 * BY;   // elements per brick row
+  auto addr = [&](int y) -> unsigned { return col_base + (unsigned)(y / BY) * brow + (y % BY) * BX; };
   // Element j + r*TC of this lane's column: a lane offset plus a uniform
   // offset per r (TC is a whole number of bricks for every schedule).
   static_assert(TC % BY == 0, "column lanes span whole bricks");

@@ -114,6 +114,7 @@ typedef struct {
   int by, bx;               /* brick rows x columns                           */
   int row_radix, row_group; /* row pass radix, fields per LDS exchange        */
   int col_radix, col_fwd_group, col_inv_group;
+  int col_width;            /* columns per column-pass block (divides bx)     */
 } af_schedule;
 int af_schedule_count(int precision, int nx, int ny);
 af_status af_schedule_get(int precision, int nx, int ny, int index, af_schedule* out);

@@ -42,9 +42,9 @@ def test_narrow_blocks_reduce_every_partial(af, orc):
     """A column block narrower than a wave (Ny = 256, W = 1, radix 8: 32
     lanes) must still fold every row-block partial of the mean-velocity RHS."""
     n = 256
-    idx = [i for i, sc in enumerate(af.schedules(64, n)) if sc[1] == 1]
+    idx = [i for i, sc in enumerate(af.schedules(64, n)) if sc[7] == 1]
     if not idx:
-        pytest.skip("no brick-width-1 schedule at 256^2")
+        pytest.skip("no width-1 column-block schedule at 256^2")
     p = af.params(n, dt=2e-4, precision=64, **JW)
     w0 = af.ic_bandlimited(n, seed=2, amp=2.0, kmax=12.0)
     M0 = (0.3 * n * n, 0.2 * n * n)
