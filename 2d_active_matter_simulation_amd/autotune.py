@@ -4,10 +4,13 @@
 For every requested (precision, N) it times each instantiated schedule on the
 GPU -- a single run, or an ensemble of `--instances` copies stepped in shared
 launches -- checks that every schedule reproduces the default schedule's state
-to rounding, and writes the fastest into schedules.txt beside the library,
-which new processes pick up automatically (af_schedule_active).
+to rounding, and writes the fastest into a table beside the library that new
+processes pick up automatically: schedules.txt (single runs) by default, or
+with --instances > 1 schedules_batched.txt (runs of the ensemble engine and
+the serving loop).
 
     python3 2d_active_matter_simulation_amd/autotune.py --sizes 256 512 1024 2048 4096
+    python3 2d_active_matter_simulation_amd/autotune.py --instances 8 --precisions 32 --sizes 256 512 1024 2048
 """
 import argparse
 import importlib
@@ -64,9 +67,12 @@ def main():
     ap.add_argument("--precisions", type=int, nargs="+", default=[32, 64])
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--instances", type=int, default=1)
-    ap.add_argument("--out", default=os.path.join(HERE, "schedules.txt"))
+    ap.add_argument("--out", default=None,
+                    help="table to write (default: schedules.txt, or schedules_batched.txt with --instances > 1)")
     ap.add_argument("--dry", action="store_true", help="print, do not write the table")
     args = ap.parse_args()
+    if args.out is None:
+        args.out = os.path.join(HERE, "schedules_batched.txt" if args.instances > 1 else "schedules.txt")
     lines = ["# prec nx ny by bx row_radix row_group col_radix col_fwd_group col_inv_group col_width  (ms/step)"]
     for prec in args.precisions:
         for n in args.sizes:

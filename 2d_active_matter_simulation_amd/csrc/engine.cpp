@@ -47,7 +47,7 @@ af_status af_engine_add(af_engine* e, const af_params* p, const double* omega_ha
   if (!e || !p || !omega_hat || !M || !slot) return fail(AF_EINVAL, "NULL argument");
   af_status st = validate(p);
   if (st != AF_OK) return st;
-  Geometry g = geometry(p);
+  Geometry g = geometry(p, true);   // the engine's tuned table (batched launches)
   if (!g.rc || !g.cc) return fail(AF_EUNSUPPORTED, "no kernel for this size");
   if (e->in_use + g.bytes > e->budget) return fail(AF_ENOMEM, "HBM budget exceeded");
   unsigned char* base = nullptr;

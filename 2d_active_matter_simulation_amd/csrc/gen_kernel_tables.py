@@ -56,7 +56,8 @@ def candidates(prec, n):
         alts = [(4, 2, 4, 3, 4, 2, 3), (4, 4, 8, 3, 8, 2, 3), (2, 4, 8, 3, 8, 2, 3), (4, 2, 8, 1, 8, 1, 1),
                 (4, 2, 8, 3, 16, 1, 1), (2, 2, 8, 3, 8, 2, 3), (4, 2, 4, 3, 8, 2, 3), (4, 2, 8, 3, 4, 2, 3)]
         narrow = [(4, 2, 4, 3, 4, 2, 3, 1), (4, 2, 8, 1, 8, 1, 1, 1), (4, 2, 8, 3, 8, 2, 3, 1),
-                  (4, 4, 8, 3, 8, 2, 3, 2), (2, 2, 8, 1, 16, 1, 1, 1), (4, 2, 4, 3, 8, 2, 3, 1)]
+                  (4, 4, 8, 3, 8, 2, 3, 2), (2, 2, 8, 1, 16, 1, 1, 1), (4, 2, 4, 3, 8, 2, 3, 1),
+                  (4, 2, 4, 3, 16, 1, 1, 1), (4, 2, 8, 3, 16, 1, 1, 1), (4, 2, 4, 3, 8, 1, 1, 1)]
     else:
         alts = [(4, 2, 4, 3, 4, 2, 3), (4, 1, 8, 3, 8, 2, 3), (2, 2, 8, 3, 8, 2, 3), (4, 2, 8, 1, 8, 1, 1),
                 (4, 2, 4, 1, 4, 1, 1), (2, 2, 4, 3, 8, 1, 1)]
@@ -110,7 +111,7 @@ const ColCfg* col_cfgs_{tag}(int* n) {{ *n = (int)(sizeof(kCol) / sizeof(kCol[0]
 const Schedule* schedules_{tag}(int* n) {{ *n = (int)(sizeof(kSched) / sizeof(kSched[0])); return kSched; }}
 ConvLaunch to_bricks_{tag}() {{ return launch_to_bricks<T>; }}
 ConvLaunch from_bricks_{tag}() {{ return launch_from_bricks<T>; }}
-
+{ifac}
 }}  // namespace af
 """
 
@@ -136,7 +137,9 @@ def gen(prec):
     cl = [f"    {{{a}, {w}, {r}, {by}, {bx}, {w * (a // r)}, launch_col<T, {a}, {w}, {r}, {by}, {bx}, {gf}, {gi}>, "
           f"{gf}, {gi}}}," for (a, w, r, by, bx, gf, gi) in sorted(cols)]
     sl = ["    {" + ", ".join(map(str, t)) + "}," for t in scheds]
-    text = HDR.format(what=f"FP{prec}", T=T, tag=tag, rows="\n".join(rl), cols="\n".join(cl), scheds="\n".join(sl))
+    ifac = "IfacLaunch ifac_init_f32() { return launch_ifac_init; }\n" if prec == 32 else ""
+    text = HDR.format(what=f"FP{prec}", T=T, tag=tag, rows="\n".join(rl), cols="\n".join(cl), scheds="\n".join(sl),
+                      ifac=ifac)
     with open(os.path.join(HERE, f"kernels_{tag}.hip"), "w") as f:
         f.write(text)
     return len(rl), len(cl), len(sl)

@@ -53,4 +53,11 @@ hipError_t launch_from_bricks(const void* src, void* dst, int nx, int ny, int by
   return hipGetLastError();
 }
 
+// Integrating-factor table of an FP32 run (written once, at set-up).
+inline hipError_t launch_ifac_init(const void* desc, int nx2, int ny, int w, hipStream_t s) {
+  hipLaunchKernelGGL(ifac_init<float>, dim3(conv_blocks((size_t)ny * nx2)), dim3(256), 0, s, (const InstDesc<float>*)desc,
+                     nx2, ny, w);
+  return hipGetLastError();
+}
+
 }  // namespace af

@@ -27,6 +27,9 @@ struct InstDesc {
   cx<T>* wm;    // mixed omega
   cx<T>* pm;    // mixed P = -i kx (lam A + beta Cy)^x
   cx<T>* qm;    // mixed Q = (lam W - beta Cx)^x
+  float* ifac;  // FP32 runs: integrating factors in column-block order, plane
+                // 0 exp(sigma dt/2), plane 1 exp(sigma dt) (FP64-evaluated,
+                // rounded once); null for FP64 runs (evaluated in the kernel)
   double* partials;   // [row tiles][4]: sum|u|^2, sum Cx, sum Cy, unused
   MeanState* mean;
   double* energy;     // ring of per-step stage-1 energies

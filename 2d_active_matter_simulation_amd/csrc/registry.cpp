@@ -1,5 +1,6 @@
-// Schedule lookup over the per-precision launch tables, with the tuned table
-// (schedules.txt next to libactflow.so, written by autotune.py) read once.
+// Schedule lookup over the per-precision launch tables, with the tuned tables
+// next to libactflow.so (written by autotune.py) read once: schedules.txt for
+// single runs, schedules_batched.txt for runs stepped by the ensemble engine.
 #include "registry.hpp"
 
 #include <dlfcn.h>
@@ -21,6 +22,7 @@ const RowCfg* row_cfgs_f32(int* n);
 const ColCfg* col_cfgs_f32(int* n);
 const Schedule* schedules_f32(int* n);
 ConvLaunch to_bricks_f32();
+IfacLaunch ifac_init_f32();
 ConvLaunch from_bricks_f32();
 const RowCfg* row_cfgs_f64(int* n);
 const ColCfg* col_cfgs_f64(int* n);
@@ -33,7 +35,8 @@ namespace {
 using Key = std::tuple<int, int, int>;   // precision, nx2, ny
 std::mutex g_mu;
 std::map<Key, const Schedule*> g_selected;
-std::map<Key, const Schedule*> g_tuned;
+std::map<Key, const Schedule*> g_tuned;           // single runs (schedules.txt)
+std::map<Key, const Schedule*> g_tuned_batched;   // ensemble engine (schedules_batched.txt)
 bool g_tuned_loaded = false;
 
 const Schedule* table(int precision, int* n) {
@@ -51,14 +54,7 @@ std::vector<const Schedule*> list_for(int precision, int nx2, int ny) {
   return out;
 }
 
-void load_tuned_locked() {
-  if (g_tuned_loaded) return;
-  g_tuned_loaded = true;
-  Dl_info info;
-  if (!dladdr((void*)&load_tuned_locked, &info) || !info.dli_fname) return;
-  std::string path(info.dli_fname);
-  const size_t slash = path.rfind('/');
-  path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/schedules.txt";
+void load_table(const std::string& path, std::map<Key, const Schedule*>& into) {
   FILE* f = std::fopen(path.c_str(), "r");
   if (!f) return;
   char line[512];
@@ -73,9 +69,21 @@ void load_tuned_locked() {
     for (const Schedule* s : list_for(prec, nx / 2, ny))
       if (s->by == v[0] && s->bx == v[1] && s->row_r == v[2] && s->row_g == v[3] && s->col_r == v[4] &&
           s->col_gf == v[5] && s->col_gi == v[6] && s->col_w == v[7])
-        g_tuned[Key(prec, nx / 2, ny)] = s;
+        into[Key(prec, nx / 2, ny)] = s;
   }
   std::fclose(f);
+}
+
+void load_tuned_locked() {
+  if (g_tuned_loaded) return;
+  g_tuned_loaded = true;
+  Dl_info info;
+  if (!dladdr((void*)&load_tuned_locked, &info) || !info.dli_fname) return;
+  std::string dir(info.dli_fname);
+  const size_t slash = dir.rfind('/');
+  dir = slash == std::string::npos ? std::string(".") : dir.substr(0, slash);
+  load_table(dir + "/schedules.txt", g_tuned);
+  load_table(dir + "/schedules_batched.txt", g_tuned_batched);
 }
 
 }  // namespace
@@ -87,12 +95,16 @@ const Schedule* schedule_at(int precision, int nx2, int ny, int index) {
   return (index >= 0 && index < (int)l.size()) ? l[index] : nullptr;
 }
 
-const Schedule* active_schedule(int precision, int nx2, int ny) {
+const Schedule* active_schedule(int precision, int nx2, int ny, bool batched) {
   std::lock_guard<std::mutex> lk(g_mu);
   const Key k(precision, nx2, ny);
   auto it = g_selected.find(k);
   if (it != g_selected.end()) return it->second;
   load_tuned_locked();
+  if (batched) {
+    auto bt = g_tuned_batched.find(k);
+    if (bt != g_tuned_batched.end()) return bt->second;
+  }
   auto jt = g_tuned.find(k);
   if (jt != g_tuned.end()) return jt->second;
   auto l = list_for(precision, nx2, ny);
@@ -132,6 +144,7 @@ const ColCfg* find_col_cfg(int precision, const Schedule* s) {
 }
 
 ConvLaunch to_bricks_launch(int precision) { return precision == 64 ? to_bricks_f64() : to_bricks_f32(); }
+IfacLaunch ifac_init_launch() { return ifac_init_f32(); }
 ConvLaunch from_bricks_launch(int precision) { return precision == 64 ? from_bricks_f64() : from_bricks_f32(); }
 
 }  // namespace af

@@ -12,6 +12,7 @@ using RowLaunch = hipError_t (*)(const void* descs, int ninst, int tiles, const 
 using ColLaunch = hipError_t (*)(const void* descs, int ninst, int tiles, int stage, const void* pool,
                                  hipStream_t s);
 using ConvLaunch = hipError_t (*)(const void* src, void* dst, int nx, int ny, int by, int bx, hipStream_t s);
+using IfacLaunch = hipError_t (*)(const void* desc, int nx2, int ny, int w, hipStream_t s);
 
 struct RowCfg {
   int nx2, rb, r, bx, threads;
@@ -31,8 +32,9 @@ struct Schedule {
 int schedule_count(int precision, int nx2, int ny);
 const Schedule* schedule_at(int precision, int nx2, int ny, int index);
 // The schedule new runs of this size use: a process-wide selection, else the
-// tuned table (schedules.txt beside the library), else the default.
-const Schedule* active_schedule(int precision, int nx2, int ny);
+// tuned table beside the library (schedules_batched.txt first for runs of the
+// ensemble engine, then schedules.txt), else the default.
+const Schedule* active_schedule(int precision, int nx2, int ny, bool batched = false);
 bool select_schedule(int precision, int nx2, int ny, int index);
 const RowCfg* find_row_cfg(int precision, const Schedule* s);
 const ColCfg* find_col_cfg(int precision, const Schedule* s);
@@ -43,6 +45,7 @@ struct LaunchEvents {
 };
 extern thread_local LaunchEvents g_launch_events;
 ConvLaunch to_bricks_launch(int precision);
+IfacLaunch ifac_init_launch();   // FP32 runs only
 ConvLaunch from_bricks_launch(int precision);
 
 }  // namespace af

@@ -84,12 +84,12 @@ static void build_pool(std::vector<unsigned char>& out, int lmax) {
 
 static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-Geometry geometry(const af_params* p) {
+Geometry geometry(const af_params* p, bool batched) {
   Geometry g;
   g.prec = p->precision;
   g.nx2 = p->nx / 2;
   g.ny = p->ny;
-  g.sched = active_schedule(g.prec, g.nx2, g.ny);
+  g.sched = active_schedule(g.prec, g.nx2, g.ny, batched);
   if (g.sched) {
     g.by = g.sched->by;
     g.bx = g.sched->bx;
@@ -110,6 +110,9 @@ Geometry geometry(const af_params* p) {
   g.off_energy = off; off += align256((size_t)kRingCap * sizeof(double));
   g.off_desc = off; off += align256(sizeof(InstDesc<double>));
   g.off_pool = off; off += align256((size_t)2 * g.lmax * g.esz);
+  g.off_ifac = off;
+  g.ifac = g.prec == 32 && g.cc && g.cc->r <= 8;   // the column pass reads the table (kTable)
+  if (g.ifac) off += align256(2 * g.nfield * sizeof(float));
   g.bytes = off;
   return g;
 }
@@ -120,6 +123,7 @@ static InstDesc<T> make_desc(const af_params* p, const Geometry& g, unsigned cha
   std::memset(&d, 0, sizeof(d));
   cx<T>** f[7] = {&d.wo, &d.acc, &d.um, &d.vm, &d.wm, &d.pm, &d.qm};
   for (int i = 0; i < 7; ++i) *f[i] = reinterpret_cast<cx<T>*>(base + g.off_field[i]);
+  d.ifac = g.ifac ? reinterpret_cast<float*>(base + g.off_ifac) : nullptr;
   d.partials = reinterpret_cast<double*>(base + g.off_partials);
   d.mean = reinterpret_cast<MeanState*>(base + g.off_mean);
   d.energy = reinterpret_cast<double*>(base + g.off_energy);
@@ -162,6 +166,10 @@ af_status init_device_block(const af_params* p, const Geometry& g, unsigned char
     e = hipMemcpyAsync(base + g.off_desc, &d, sizeof(d), hipMemcpyHostToDevice, stream);
   }
   if (e != hipSuccess) return hip_fail(e, "upload descriptor");
+  if (g.ifac) {
+    e = ifac_init_launch()(base + g.off_desc, g.nx2, g.ny, g.cc->w, stream);
+    if (e != hipSuccess) return hip_fail(e, "integrating factors");
+  }
   MeanState ms;
   std::memset(&ms, 0, sizeof(ms));
   ms.nan_step = -1;

@@ -25,6 +25,8 @@ struct Geometry {
   int row_tiles = 0, col_tiles = 0;
   size_t off_field[7] = {0, 0, 0, 0, 0, 0, 0};  // wo, acc, um, vm, wm, pm, qm
   size_t off_partials = 0, off_mean = 0, off_energy = 0, off_desc = 0, off_pool = 0;
+  size_t off_ifac = 0;   // FP32 integrating-factor table (2 float planes) when ifac
+  bool ifac = false;     // FP32 with a column radix <= 8
   size_t bytes = 0;
 };
 
@@ -32,7 +34,7 @@ extern thread_local std::string g_last_error;
 af_status fail(af_status s, const std::string& msg);
 af_status hip_fail(hipError_t e, const char* what);
 af_status validate(const af_params* p);
-Geometry geometry(const af_params* p);
+Geometry geometry(const af_params* p, bool batched = false);
 af_status init_device_block(const af_params* p, const Geometry& g, unsigned char* base, hipStream_t stream);
 af_status load_state(const Geometry& g, unsigned char* base, const double* omega_hat, const double M[2],
                      hipStream_t stream);

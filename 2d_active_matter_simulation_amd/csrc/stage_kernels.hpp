@@ -66,8 +66,10 @@ struct BufField {
       : rs(__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(p), 0, 0x7fffffff, 0x00020000)),
         base((gptr<char>)glob(const_cast<T*>(p))) {}
   __device__ __forceinline__ T ld(unsigned vbyte, unsigned sbyte) const {
-    static_assert(sizeof(T) == 8 || sizeof(T) == 16, "complex float or double");
-    if constexpr (sizeof(T) == 8)
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "float, complex float or double");
+    if constexpr (sizeof(T) == 4)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, vbyte, sbyte, 0));
+    else if constexpr (sizeof(T) == 8)
       return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, vbyte, sbyte, 0));
     else
       return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(rs, vbyte, sbyte, 0));
@@ -463,10 +465,28 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 #pragma unroll
       for (int r = 0; r < R; ++r) av[r] = ld((stage == 1) ? d.wo : d.acc, r);
     };
+    // FP32 with radix <= 8: integrating factors from the run's table
+    // (FP64-evaluated, rounded once), loaded with omega_old; otherwise (FP64,
+    // or radix 16 where the block is latency bound and the extra loads cost
+    // more than the FP64 exp they save) evaluated in the update.
+    constexpr bool kTable = sizeof(T) == 4 && R <= 8;
+    float ef[2][R];
     auto load_wv = [&]() {
       if (stage == 2 || stage == 3) {
 #pragma unroll
         for (int r = 0; r < R; ++r) wv[r] = ld(d.wo, r);
+      }
+      if constexpr (kTable) {
+        if (stage <= 3) {
+          const BufField<float> tf(d.ifac);
+          const unsigned vbf = ((unsigned)tile * NY * W + (unsigned)t) * 4u, rsf = (unsigned)TC * W * 4u;
+          const unsigned plane = (unsigned)d.ny * (unsigned)nx2 * 4u;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            ef[0][r] = tf.ld(vbf, (unsigned)r * rsf);
+            if (stage != 2) ef[1][r] = tf.ld(vbf, plane + (unsigned)r * rsf);
+          }
+        }
       }
     };
     // Hermitian projection of the kx = 0 column (reality_func, .cu:1237-1242):
@@ -525,10 +545,15 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
       const cx<T> nh = cscale(keepf, pq[0][r] + cmuli(cscale((T)(-kyd), pq[1][r])));
       T E = (T)1, E2 = (T)1;
       if (stage <= 3) {
-        const double sig = d.c0 + d.c2 * k2 + d.c4 * (k2 * k2);
-        const double e2d = exp(sig * dt * 0.5);
-        E2 = (T)e2d;
-        E = (T)(e2d * e2d);
+        if constexpr (kTable) {
+          E2 = ef[0][r];
+          E = ef[1][r];
+        } else {
+          const double sig = d.c0 + d.c2 * k2 + d.c4 * (k2 * k2);
+          const double e2d = exp(sig * dt * 0.5);
+          E2 = (T)e2d;
+          E = (T)(e2d * e2d);
+        }
       }
       cx<T> a = (stage == 1) ? cscale(E, av[r]) : av[r];
       cx<T> s;
@@ -641,6 +666,29 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 __device__ __forceinline__ size_t brick_index(int y, int x, int nx2, int by, int bx) {
   return ((size_t)(y / by) * (nx2 / bx) + x / bx) * (size_t)(by * bx) + (y % by) * bx + x % bx;
 }
+// Integrating factors of every mode of an FP32 run, evaluated in FP64 exactly
+// as the column pass of an FP64 run does and rounded once to FP32.  Stored in
+// the order the column pass reads them: plane 0 exp(sigma dt/2), plane 1
+// exp(sigma dt), each [column tile][y][column in tile] for tiles of W columns,
+// so a column block reads one contiguous run.  Instantiated for T = float.
+template <class T>
+__global__ void ifac_init(const InstDesc<T>* __restrict__ dp, int nx2, int ny, int w) {
+  const InstDesc<T>& d = *dp;
+  const size_t n = (size_t)ny * nx2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int y = (int)(i / nx2), x = (int)(i % nx2);
+    const int nys = (y <= ny / 2) ? y : y - ny;
+    const double kxd = d.tkx * (double)x;
+    const double kyd = d.tky * (double)nys;
+    const double k2 = kxd * kxd + kyd * kyd;
+    const double sig = d.c0 + d.c2 * k2 + d.c4 * (k2 * k2);
+    const double e2d = exp(sig * d.dt * 0.5);
+    const size_t b = ((size_t)(x / w) * ny + y) * w + x % w;
+    d.ifac[b] = (float)e2d;
+    d.ifac[n + b] = (float)(e2d * e2d);
+  }
+}
+
 template <class T>
 __global__ void to_bricks(const double* __restrict__ src, cx<T>* __restrict__ dst, int nx, int ny, int by,
                           int bx) {

@@ -122,6 +122,26 @@ def ensemble_leg(af, steps=200, seed=7):
             "hbm_frac_50_transfers": 50 * fb / (wall / steps) / 1e9 / HBM_PEAK_GBS}
 
 
+def batched_leg(af, n, count, steps, seed=3):
+    """Config 3 sizes stepped as a batch: `count` FP32 instances of n x n with
+    lambda0 cycling over {2, 3.5, 5} (other parameters as the headline), in
+    shared launches of the ensemble engine (its tuned batched schedule)."""
+    eng = af.Engine(200 << 30)
+    for i in range(count):
+        p = af.params(n, dt=1e-4 if n <= 1024 else 5e-5, lam=(2.0, 3.5, 5.0)[i % 3], alpha=-0.8, beta=0.01,
+                      gamma0=-0.045, gamma2=2.5e-5, precision=32)
+        eng.add(p, af.ic_bandlimited(n, seed=seed + i, amp=1e-3, kmax=10.0))
+    eng.step(5)
+    t0 = time.perf_counter()
+    eng.step(steps)
+    wall = time.perf_counter() - t0
+    eng.close()
+    fb = field_bytes(n, 32) * count
+    return {"grid": [n, n], "instances": count, "steps": steps, "value": count * n * n * steps / wall,
+            "ms_per_step": wall / steps * 1e3,
+            "hbm_frac_50_transfers": 50 * fb / (wall / steps) / 1e9 / HBM_PEAK_GBS}
+
+
 def serving_leg(af, rate, window=0.5, seed=11, horizon_scale=0.1):
     """Config 5: Poisson arrivals at `rate`/s for `window` s; N from
     {128..2048}^2 with weights 0.3/0.3/0.2/0.15/0.05; horizon U{1k..50k}
@@ -179,7 +199,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip FP64 / 2048^2 / 4096^2 lines")
-    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-instance ensemble leg")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble and batched legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the served-workload leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -233,6 +253,13 @@ def main():
     }
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(af)
+        out["batched"] = {}
+        for n, cnt, steps in ((1024, 8, 400), (2048, 4, 100), (2048, 8, 60)):
+            b = batched_leg(af, n, cnt, steps)
+            fp64 = extra.get(f"{n}_fp64")
+            if fp64:
+                b["over_fp64_single_run"] = b["value"] / fp64["value"]
+            out["batched"][f"{n}x{cnt}"] = b
     if not args.no_serving:
         out["serving"] = [serving_leg(af, r) for r in (10, 50, 200)]
     if not args.no_cpu:

@@ -108,8 +108,10 @@ size_t af_run_device_bytes(const af_params* p);
  * A schedule fixes the brick shape of the fields in HBM and, per pass, the
  * radix and how many fields share an LDS exchange.  Index 0 is the built-in
  * default; runs take the schedule active for their size when created
- * (af_schedule_select, else the tuned table schedules.txt written by
- * autotune.py next to the library, else the default). */
+ * (af_schedule_select, else the tuned tables written by autotune.py next to
+ * the library -- schedules_batched.txt first for runs admitted to an engine,
+ * then schedules.txt -- else the default).  af_schedule_active reports the
+ * single-run choice. */
 typedef struct {
   int by, bx;               /* brick rows x columns                           */
   int row_radix, row_group; /* row pass radix, fields per LDS exchange        */
@@ -133,7 +135,9 @@ af_status af_engine_add(af_engine* e, const af_params* p, const double* omega_ha
 af_status af_engine_remove(af_engine* e, int slot);
 /* Advance every resident run by nsteps steps, all runs of one size class in
  * the same kernel launches.  Per-run results are bitwise identical to stepping
- * the run alone. */
+ * the run alone in an engine (and to af_run_step whenever the engine's tuned
+ * schedule for the size is the single-run one; otherwise they agree to the
+ * rounding of the precision). */
 af_status af_engine_step(af_engine* e, int nsteps);
 /* Advance only the listed slots (same batching within the list). */
 af_status af_engine_step_slots(af_engine* e, const int* slots, int nslots, int nsteps);

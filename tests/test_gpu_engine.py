@@ -23,20 +23,30 @@ def _cases(af):
 
 
 def test_alone_equals_mix_bitwise(af):
+    """Each run stepped in a mix equals the same run stepped alone in an
+    engine, bitwise; and equals the single-run API (bitwise at these sizes,
+    where both use the same schedule)."""
     cases = _cases(af)
     eng = af.Engine(1 << 34)
     slots = [eng.add(p, w, M) for p, w, M in cases]
     eng.step(17)
     eng.step(13)
     for (p, w, M), s in zip(cases, slots):
-        run = af.Run(p)
-        run.set_state(w, M)
-        e_alone = run.step(30, energy=True)
-        wa, ma = run.get_state()
+        solo = af.Engine(1 << 34)
+        s1 = solo.add(p, w, M)
+        solo.step(30)
+        wa, ma = solo.get_state(s1)
         wm, mm = eng.get_state(s)
         assert np.array_equal(wa, wm) and np.array_equal(ma, mm), s
-        assert np.array_equal(eng.energy(s, 30), e_alone)
+        assert np.array_equal(eng.energy(s, 30), solo.energy(s1, 30))
         assert eng.steps_done(s) == 30
+        solo.close()
+        run = af.Run(p)
+        run.set_state(w, M)
+        e_run = run.step(30, energy=True)
+        wr, mr = run.get_state()
+        assert np.array_equal(wr, wm) and np.array_equal(mr, mm), s
+        assert np.array_equal(eng.energy(s, 30), e_run)
         run.close()
     eng.close()
 
