@@ -39,8 +39,19 @@ def field_bytes(n, prec):
 #   row pass: reads v_x, v_y, omega (mixed), writes P, Q             -> 5
 #   column pass: reads P, Q, omega_old (stages 1-3), acc (stages 2-4);
 #                writes acc (or the new state) and v_x, v_y, omega   -> 7, 8, 8, 7
+#                (+ the FP32 integrating-factor table, see col_fields)
 ROW_FIELDS = 5.0
 COL_FIELDS = (7 + 8 + 8 + 7) / 4.0
+
+
+def col_fields(af, n, prec):
+    """Column-pass transfers per launch, plus the integrating-factor table an
+    FP32 run reads when its column radix is <= 8 (4 B per mode per plane:
+    2 planes at stages 1 and 3, 1 at stage 2, in units of one FP32 field)."""
+    extra = 0.0
+    if prec == 32 and af.active_schedule(prec, n)[4] <= 8:
+        extra = (2 + 1 + 2 + 0) / 4.0 * 0.5
+    return COL_FIELDS + extra
 
 
 def read_traffic(workload_tag):
@@ -79,9 +90,10 @@ def run_workload(af, n, prec, steps, warmup, seed=1):
                 prof_ms_per_step=tot_ms / kprof)
 
 
-def roofline(r):
+def roofline(af, r):
     fb = field_bytes(r["n"], r["prec"])
-    kernels = {"col_pass": (COL_FIELDS * fb, r["col_ms"]), "row_pass": (ROW_FIELDS * fb, r["row_ms"])}
+    kernels = {"col_pass": (col_fields(af, r["n"], r["prec"]) * fb, r["col_ms"]),
+               "row_pass": (ROW_FIELDS * fb, r["row_ms"])}
     name = max(kernels, key=lambda k: kernels[k][1])
     alg, ms = kernels[name]
     achieved = alg / (ms * 1e-3) / 1e9
@@ -210,7 +222,7 @@ def main():
     af.lib()
 
     head = run_workload(af, 1024, 32, args.steps, args.warmup)
-    name, alg, kms, achieved = roofline(head)
+    name, alg, kms, achieved = roofline(af, head)
     traffic = read_traffic("1024_fp32")
     tr_bytes = None
     if traffic and f"{name}_bytes_per_launch" in traffic:
@@ -220,7 +232,7 @@ def main():
         for n, prec, steps in ((1024, 64, max(200, args.steps // 4)), (2048, 32, max(100, args.steps // 8)),
                                (2048, 64, max(50, args.steps // 16)), (4096, 32, max(20, args.steps // 32))):
             r = run_workload(af, n, prec, steps, min(args.warmup, 20))
-            nm, a, ms, ach = roofline(r)
+            nm, a, ms, ach = roofline(af, r)
             extra[f"{n}_fp{prec}"] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "dominant": nm,
                                       "dominant_ms": ms, "achieved_GBs": ach,
                                       "frac": ach / HBM_PEAK_GBS,

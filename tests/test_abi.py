@@ -62,3 +62,30 @@ def test_create_rejects_bad_sizes(af):
             assert e.status in (af.AF_EINVAL, af.AF_EUNSUPPORTED)
         else:
             raise AssertionError("accepted bad parameters")
+
+
+def test_tuned_tables_name_instantiated_schedules(af):
+    """Every line of the shipped tuned tables (schedules.txt for single runs,
+    schedules_batched.txt for the engine) names a schedule the library has
+    kernels for, so no tuned size silently falls back to the default."""
+    pkg = os.path.join(ROOT, "2d_active_matter_simulation_amd")
+    for name in ("schedules.txt", "schedules_batched.txt"):
+        path = os.path.join(pkg, name)
+        assert os.path.exists(path), name
+        for line in open(path):
+            body = line.split("#")[0].split()
+            if not body:
+                continue
+            v = [int(x) for x in body]
+            prec, nx, ny, sched = v[0], v[1], v[2], tuple(v[3:])
+            if len(sched) == 7:
+                sched = sched + (sched[1],)
+            assert sched in af.schedules(prec, nx, ny), (name, line)
+
+
+def test_schedule_struct_fields(af):
+    """af_schedule carries the column-block width (divides the brick width)."""
+    for prec in (32, 64):
+        for sc in af.schedules(prec, 1024):
+            by, bx, rr, rg, cr, gf, gi, cw = sc
+            assert bx % cw == 0 and rr in (4, 8, 16) and cr in (4, 8, 16)

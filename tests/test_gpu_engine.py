@@ -85,3 +85,26 @@ def test_hbm_budget_admission(af):
     eng.add(p, w)
     assert eng.count == 2
     eng.close()
+
+
+def test_engine_schedule_matches_single_run_to_rounding(af):
+    """At 1024^2 FP32 the engine's tuned schedule differs from the single-run
+    one (radix, fields per exchange); both integrate the same equation, so
+    the states agree to FP32 rounding after a few steps (relative L2 with
+    the half-spectrum weights, tolerance 1e-5)."""
+    from conftest import rel_l2
+    n = 1024
+    p = af.params(n, dt=1e-4, precision=32, lam=3.5, alpha=-0.8, beta=0.01, gamma0=-0.045, gamma2=2.5e-5)
+    w0 = af.ic_bandlimited(n, seed=5, amp=1.0, kmax=10.0)
+    eng = af.Engine(1 << 34)
+    s = eng.add(p, w0, (0.0, 0.0))
+    eng.step(10)
+    we, me = eng.get_state(s)
+    eng.close()
+    run = af.Run(p)
+    run.set_state(w0)
+    run.step(10)
+    wr, mr = run.get_state()
+    run.close()
+    assert rel_l2(we, wr) <= 1e-5
+    assert np.allclose(me, mr, rtol=1e-5, atol=1e-6 * n * n)
