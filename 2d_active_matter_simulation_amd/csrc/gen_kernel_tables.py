@@ -60,7 +60,8 @@ def candidates(prec, n):
                   (4, 2, 4, 3, 16, 1, 1, 1), (4, 2, 8, 3, 16, 1, 1, 1), (4, 2, 4, 3, 8, 1, 1, 1),
                   (2, 2, 8, 1, 8, 1, 1, 2), (2, 2, 8, 1, 8, 1, 1, 1), (2, 2, 4, 3, 8, 1, 1, 2),
                   (2, 2, 4, 1, 4, 1, 1, 2), (4, 4, 8, 1, 16, 1, 1, 4), (4, 4, 8, 1, 8, 1, 1, 4),
-                  (4, 4, 8, 3, 16, 1, 1, 4)]
+                  (4, 4, 8, 3, 16, 1, 1, 4), (4, 2, 16, 1, 8, 1, 1, 2), (4, 2, 16, 3, 8, 1, 1, 2),
+                  (4, 2, 16, 1, 4, 2, 3, 2)]
     else:
         alts = [(4, 2, 4, 3, 4, 2, 3), (4, 1, 8, 3, 8, 2, 3), (2, 2, 8, 3, 8, 2, 3), (4, 2, 8, 1, 8, 1, 1),
                 (4, 2, 4, 1, 4, 1, 1), (2, 2, 4, 3, 8, 1, 1)]
