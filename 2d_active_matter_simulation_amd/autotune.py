@@ -73,7 +73,7 @@ def main():
     args = ap.parse_args()
     if args.out is None:
         args.out = os.path.join(HERE, "schedules_batched.txt" if args.instances > 1 else "schedules.txt")
-    lines = ["# prec nx ny by bx row_radix row_group col_radix col_fwd_group col_inv_group col_width  (ms/step)"]
+    lines = ["# prec nx ny by bx row_radix row_group col_radix col_fwd_group col_inv_group col_width lds_double  (ms/step)"]
     for prec in args.precisions:
         for n in args.sizes:
             scheds = af.schedules(prec, n)

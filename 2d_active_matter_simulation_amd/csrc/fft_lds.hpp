@@ -156,7 +156,11 @@ struct TwLayout {
 // barriers).  The first LDS access is a store after radix pass 1, so the
 // caller must place a barrier before the call if other lanes may still be
 // reading the same LDS areas from an earlier use.
-template <int L, int R, int DIR, int NF, int W, class T>
+// BUF (elements): offset of a second set of LDS areas, or 0.  With two sets
+// the exchanges alternate between them (exchange e writes set e % 2), so the
+// barrier that keeps a pass's stores from overwriting data other lanes are
+// still loading is not needed: one barrier per exchange instead of two.
+template <int L, int R, int DIR, int NF, int W, class T, int BUF = 0>
 struct FftMulti {
   static_assert((L & (L - 1)) == 0 && (R & (R - 1)) == 0 && R <= L && R <= 16, "power-of-two sizes");
   static constexpr int NP = ilog(L, R);       // full radix-R passes
@@ -182,7 +186,7 @@ struct FftMulti {
       constexpr int Ns = ipow(R, P);
       if constexpr (P > 0) {
         __syncthreads();
-        load<Ns / R>(v, j, c, lds);
+        load<Ns / R>(v, j, c, lds + ((P - 1) % 2) * BUF);
         const int jm = j & (Ns - 1);
         const cx<T>* twp = tw + (TwLayout<L, R>::off(P) - 1) + jm * (R - 1);
 #pragma unroll
@@ -196,19 +200,20 @@ struct FftMulti {
       for (int f = 0; f < NF; ++f) Dft<R, DIR, T>::run(v[f]);
       constexpr bool last = (P == NP - 1) && (RL == 1);
       if constexpr (!last) {
-        if constexpr (P > 0) __syncthreads();
+        if constexpr (P > 0 && BUF == 0) __syncthreads();
         const int base = (j / Ns) * Ns * R + (j & (Ns - 1));
         const int a = pad(base * W + c, Ns);
+        cx<T>* const st = lds + (P % 2) * BUF;
 #pragma unroll
         for (int f = 0; f < NF; ++f)
 #pragma unroll
-          for (int r = 0; r < R; ++r) lds[f * AREA + a + lds_pad<W, R, ESZ>(r * Ns * W, Ns)] = v[f][r];
+          for (int r = 0; r < R; ++r) st[f * AREA + a + lds_pad<W, R, ESZ>(r * Ns * W, Ns)] = v[f][r];
       }
       pass<P + 1>(v, j, c, lds, tw);
     } else if constexpr (RL > 1) {
       constexpr int Nsp = ipow(R, NP - 1);
       __syncthreads();
-      load<Nsp>(v, j, c, lds);
+      load<Nsp>(v, j, c, lds + ((NP - 1) % 2) * BUF);
       // R/RL butterflies of radix RL on elements m = q + (R/RL) r'
 #pragma unroll
       for (int q = 0; q < R / RL; ++q) {
@@ -238,7 +243,8 @@ struct FftMulti {
 template <int L, int R, int DIR, int NF, class T, class Lds>
 __device__ __forceinline__ void fft_multi(cx<T> (&v)[NF][R], int j, Lds& lds, const cx<T>* tw) {
   static_assert(Lds::kL == L && Lds::kR == R, "LDS areas laid out for this transform");
-  FftMulti<L, R, DIR, NF, Lds::kW, T>::template pass<0>(v, j, lds.c, lds.base, tw);
+  static_assert(NF <= Lds::kFields, "fields fit the LDS areas of one set");
+  FftMulti<L, R, DIR, NF, Lds::kW, T, Lds::kBuf>::template pass<0>(v, j, lds.c, lds.base, tw);
 }
 
 }  // namespace af

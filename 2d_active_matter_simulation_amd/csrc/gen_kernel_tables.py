@@ -24,16 +24,16 @@ def lds_area(L, W, R):
     return L * W + (L * W) // R + 1
 
 
-def lds_row(nx2, rb, R, G, esz):
-    return (G * lds_area(nx2, rb, R) + 2 * nx2) * esz
+def lds_row(nx2, rb, R, G, esz, pp=0):
+    return ((2 if pp else 1) * G * lds_area(nx2, rb, R) + 2 * nx2) * esz
 
 
-def lds_col(ny, W, R, GF, GI, esz):
-    return (max(GF, GI) * lds_area(ny, W, R) + ny) * esz
+def lds_col(ny, W, R, GF, GI, esz, pp=0):
+    return ((2 if pp else 1) * max(GF, GI) * lds_area(ny, W, R) + ny) * esz
 
 
 def default_schedule(prec, nx, ny):
-    """(by, bx, row_r, row_g, col_r, col_gf, col_gi, col_w)"""
+    """(by, bx, row_r, row_g, col_r, col_gf, col_gi, col_w, pp)"""
     nx2 = nx // 2
     if prec == 32:
         by = 2 if nx2 >= 2048 else 4
@@ -45,7 +45,7 @@ def default_schedule(prec, nx, ny):
         bx = 1 if ny >= 4096 else 2
         row = (4, 3) if nx2 == 16 else ((8, 1) if nx2 >= 1024 else (8, 3))
         col = (8, 1, 1) if ny >= 2048 else (8, 2, 3)
-    return (by, bx) + row + col + (bx,)
+    return (by, bx) + row + col + (bx, 0)
 
 
 def candidates(prec, n):
@@ -66,7 +66,14 @@ def candidates(prec, n):
         alts = [(4, 2, 4, 3, 4, 2, 3), (4, 1, 8, 3, 8, 2, 3), (2, 2, 8, 3, 8, 2, 3), (4, 2, 8, 1, 8, 1, 1),
                 (4, 2, 4, 1, 4, 1, 1), (2, 2, 4, 3, 8, 1, 1)]
         narrow = [(4, 2, 4, 3, 4, 2, 3, 1), (4, 2, 8, 3, 8, 2, 3, 1), (4, 2, 8, 1, 8, 1, 1, 1)]
-    alts = [a + (a[1],) for a in alts] + narrow
+    alts = [a + (a[1], 0) for a in alts] + [a + (0,) for a in narrow]
+    # double-buffered LDS exchanges for the single-run schedules of the small sizes
+    # (one block per CU there, so the doubled LDS costs no occupancy)
+    if prec == 32:
+        alts += [(4, 2, 4, 3, 4, 2, 3, 2, 1), (4, 2, 4, 3, 4, 2, 3, 1, 1), (4, 2, 8, 3, 8, 2, 3, 2, 1),
+                 (4, 2, 16, 1, 8, 1, 1, 2, 1), (4, 2, 8, 1, 8, 1, 1, 2, 1)]
+    else:
+        alts += [(4, 2, 4, 3, 4, 2, 3, 2, 1), (4, 1, 8, 3, 8, 2, 3, 1, 1), (4, 2, 8, 1, 8, 1, 1, 1, 1)]
     for s in alts:
         if s not in out and feasible(prec, n, n, s):
             out.append(s)
@@ -74,7 +81,7 @@ def candidates(prec, n):
 
 
 def feasible(prec, nx, ny, s):
-    by, bx, rr, rg, cr, gf, gi, cw = s
+    by, bx, rr, rg, cr, gf, gi, cw, pp = s
     esz = 8 if prec == 32 else 16
     nx2 = nx // 2
     if rr > nx2 or cr > ny:
@@ -84,19 +91,21 @@ def feasible(prec, nx, ny, s):
         return False
     if by * tr > 512 or cw * tc > 512:
         return False
-    if lds_row(nx2, by, rr, rg, esz) > LDS_MAX or lds_col(ny, cw, cr, gf, gi, esz) > LDS_MAX:
+    if lds_row(nx2, by, rr, rg, esz, pp) > LDS_MAX or lds_col(ny, cw, cr, gf, gi, esz, pp) > LDS_MAX:
         return False
     return True
 
 
-HDR = """// {what} instantiations of the stage kernels and the schedule table.
+HDR = """// {what} schedule and launch tables (the kernels are instantiated in
+// kernels_{tag}_k*.hip, compiled in parallel).
 // Generated by gen_kernel_tables.py -- edit that script, not this file.
-// Row kernel:    (Nx/2, RB = brick height, R, BX = brick width, G)
-// Column kernel: (Ny, W = columns per block, R, BY, BX = brick height/width, GF, GI)
-// Schedule:      (Nx/2, Ny, BY, BX, row R, row G, col R, col GF, col GI, col W, default)
+// Row kernel:    (Nx/2, RB = brick height, R, BX = brick width, G, PP)
+// Column kernel: (Ny, W = columns per block, R, BY, BX = brick height/width, GF, GI, PP)
+// Schedule:      (Nx/2, Ny, BY, BX, row R, row G, col R, col GF, col GI, col W, PP, default)
 #include "kernels_inst.hpp"
 
 namespace af {{
+{externs}
 namespace {{
 using T = {T};
 const RowCfg kRow[] = {{
@@ -133,20 +142,54 @@ def gen(prec):
                 for s in candidates(prec, nx)[1:]:
                     scheds.append((nx // 2, ny) + s + (0,))
     rows, cols = set(), set()
-    for (nx2, ny, by, bx, rr, rg, cr, gf, gi, cw, _) in scheds:
-        rows.add((nx2, by, rr, bx, rg))
-        cols.add((ny, cw, cr, by, bx, gf, gi))
-    rl = [f"    {{{a}, {b}, {r}, {c}, {b * (a // r)}, launch_row<T, {a}, {b}, {r}, {c}, {g}>, {g}}},"
-          for (a, b, r, c, g) in sorted(rows)]
-    cl = [f"    {{{a}, {w}, {r}, {by}, {bx}, {w * (a // r)}, launch_col<T, {a}, {w}, {r}, {by}, {bx}, {gf}, {gi}>, "
-          f"{gf}, {gi}}}," for (a, w, r, by, bx, gf, gi) in sorted(cols)]
+    for (nx2, ny, by, bx, rr, rg, cr, gf, gi, cw, pp, _) in scheds:
+        rows.add((nx2, by, rr, bx, rg, pp))
+        cols.add((ny, cw, cr, by, bx, gf, gi, pp))
+    rl = [f"    {{{a}, {b}, {r}, {c}, {b * (a // r)}, launch_row<T, {a}, {b}, {r}, {c}, {g}, {pp}>, {g}, {pp}}},"
+          for (a, b, r, c, g, pp) in sorted(rows)]
+    cl = [f"    {{{a}, {w}, {r}, {by}, {bx}, {w * (a // r)}, launch_col<T, {a}, {w}, {r}, {by}, {bx}, {gf}, {gi}, {pp}>, "
+          f"{gf}, {gi}, {pp}}}," for (a, w, r, by, bx, gf, gi, pp) in sorted(cols)]
     sl = ["    {" + ", ".join(map(str, t)) + "}," for t in scheds]
     ifac = "IfacLaunch ifac_init_f32() { return launch_ifac_init; }\n" if prec == 32 else ""
+    # explicit instantiations, spread over parts of similar weight (a column
+    # launcher instantiates 5 stage kernels, a row launcher 1)
+    row_sig = "(const void*, int, int, const void*, hipStream_t)"
+    col_sig = "(const void*, int, int, int, const void*, hipStream_t)"
+    insts = [(1, f"hipError_t launch_row<{T}, {a}, {b}, {r}, {c}, {g}, {pp}>{row_sig}")
+             for (a, b, r, c, g, pp) in sorted(rows)]
+    insts += [(5, f"hipError_t launch_col<{T}, {a}, {w}, {r}, {by}, {bx}, {gf}, {gi}, {pp}>{col_sig}")
+              for (a, w, r, by, bx, gf, gi, pp) in sorted(cols)]
+    nparts = PARTS[prec]
+    parts = [[] for _ in range(nparts)]
+    load = [0] * nparts
+    for wgt, sig in sorted(insts, key=lambda x: -x[0]):
+        k = load.index(min(load))
+        parts[k].append(sig)
+        load[k] += wgt
+    externs = "\n".join(f"extern template {sig};" for _, sig in insts)
     text = HDR.format(what=f"FP{prec}", T=T, tag=tag, rows="\n".join(rl), cols="\n".join(cl), scheds="\n".join(sl),
-                      ifac=ifac)
+                      ifac=ifac, externs=externs)
     with open(os.path.join(HERE, f"kernels_{tag}.hip"), "w") as f:
         f.write(text)
+    for old in os.listdir(HERE):
+        if old.startswith(f"kernels_{tag}_k") and old.endswith(".hip"):
+            os.remove(os.path.join(HERE, old))
+    for k, sigs in enumerate(parts):
+        body = "\n".join(f"template {sig};" for sig in sigs)
+        with open(os.path.join(HERE, f"kernels_{tag}_k{k}.hip"), "w") as f:
+            f.write(PART.format(what=f"FP{prec}", k=k, n=nparts, body=body))
     return len(rl), len(cl), len(sl)
+
+
+PARTS = {32: 8, 64: 4}
+PART = """// {what} stage-kernel instantiations, part {k} of {n}.
+// Generated by gen_kernel_tables.py -- edit that script, not this file.
+#include "kernels_inst.hpp"
+
+namespace af {{
+{body}
+}}  // namespace af
+"""
 
 
 if __name__ == "__main__":

@@ -1,0 +1,28 @@
+// FP64 stage-kernel instantiations, part 0 of 4.
+// Generated by gen_kernel_tables.py -- edit that script, not this file.
+#include "kernels_inst.hpp"
+
+namespace af {
+template hipError_t launch_col<double, 32, 2, 8, 2, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_col<double, 128, 2, 8, 2, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_col<double, 256, 1, 8, 4, 1, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_col<double, 256, 2, 4, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_col<double, 256, 2, 8, 2, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_col<double, 512, 1, 8, 4, 1, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_col<double, 512, 1, 8, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_col<double, 512, 2, 8, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_col<double, 1024, 1, 4, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_col<double, 1024, 1, 8, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_col<double, 1024, 2, 8, 2, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_col<double, 2048, 2, 8, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_row<double, 16, 4, 4, 1, 3, 0>(const void*, int, int, const void*, hipStream_t);
+template hipError_t launch_row<double, 64, 4, 8, 1, 3, 0>(const void*, int, int, const void*, hipStream_t);
+template hipError_t launch_row<double, 128, 4, 4, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
+template hipError_t launch_row<double, 128, 4, 8, 1, 3, 1>(const void*, int, int, const void*, hipStream_t);
+template hipError_t launch_row<double, 256, 2, 4, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
+template hipError_t launch_row<double, 256, 4, 4, 2, 3, 1>(const void*, int, int, const void*, hipStream_t);
+template hipError_t launch_row<double, 256, 4, 8, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
+template hipError_t launch_row<double, 512, 4, 4, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
+template hipError_t launch_row<double, 512, 4, 8, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
+template hipError_t launch_row<double, 1024, 4, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
+}  // namespace af

@@ -14,24 +14,24 @@ inline void launch(K kernel, dim3 grid, dim3 block, hipStream_t s, A... args) {
   hipExtLaunchKernelGGL(kernel, grid, block, 0, s, g_launch_events.start, g_launch_events.stop, 0, args...);
 }
 
-template <class T, int NX2, int RB, int R, int BX, int G>
+template <class T, int NX2, int RB, int R, int BX, int G, int PP>
 hipError_t launch_row(const void* descs, int ninst, int tiles, const void* pool, hipStream_t s) {
-  launch(row_pass<T, NX2, RB, R, BX, G>, dim3(ninst * tiles), dim3(RB * (NX2 / R)), s, (const InstDesc<T>*)descs,
+  launch(row_pass<T, NX2, RB, R, BX, G, PP>, dim3(ninst * tiles), dim3(RB * (NX2 / R)), s, (const InstDesc<T>*)descs,
          tiles, (const cx<T>*)pool);
   return hipGetLastError();
 }
 
-template <class T, int NY, int W, int R, int BY, int BX, int GF, int GI>
+template <class T, int NY, int W, int R, int BY, int BX, int GF, int GI, int PP>
 hipError_t launch_col(const void* descs, int ninst, int tiles, int stage, const void* pool, hipStream_t s) {
   const dim3 grid(ninst * tiles), block(W * (NY / R));
   const InstDesc<T>* dd = (const InstDesc<T>*)descs;
   const cx<T>* pp = (const cx<T>*)pool;
   switch (stage) {
-    case 0: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, 0>, grid, block, s, dd, tiles, pp); break;
-    case 1: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, 1>, grid, block, s, dd, tiles, pp); break;
-    case 2: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, 2>, grid, block, s, dd, tiles, pp); break;
-    case 3: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, 3>, grid, block, s, dd, tiles, pp); break;
-    case 4: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, 4>, grid, block, s, dd, tiles, pp); break;
+    case 0: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, PP, 0>, grid, block, s, dd, tiles, pp); break;
+    case 1: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, PP, 1>, grid, block, s, dd, tiles, pp); break;
+    case 2: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, PP, 2>, grid, block, s, dd, tiles, pp); break;
+    case 3: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, PP, 3>, grid, block, s, dd, tiles, pp); break;
+    case 4: launch(col_pass<T, NY, W, R, BY, BX, GF, GI, PP, 4>, grid, block, s, dd, tiles, pp); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

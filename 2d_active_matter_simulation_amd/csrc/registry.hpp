@@ -17,15 +17,15 @@ using IfacLaunch = hipError_t (*)(const void* desc, int nx2, int ny, int w, hipS
 struct RowCfg {
   int nx2, rb, r, bx, threads;
   RowLaunch launch;
-  int g;
+  int g, pp;   // pp: double-buffered LDS exchanges
 };
 struct ColCfg {
   int ny, w, r, by, bx, threads;   // w: columns per block (divides the brick width bx)
   ColLaunch launch;
-  int gf, gi;
+  int gf, gi, pp;
 };
 struct Schedule {
-  int nx2, ny, by, bx, row_r, row_g, col_r, col_gf, col_gi, col_w, is_default;
+  int nx2, ny, by, bx, row_r, row_g, col_r, col_gf, col_gi, col_w, pp, is_default;
 };
 
 // Schedules instantiated for (precision, nx2, ny); index 0 is the default.

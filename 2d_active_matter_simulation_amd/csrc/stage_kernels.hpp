@@ -108,11 +108,14 @@ __device__ __forceinline__ int mask_ymax(int x, int nx, int ny, int mode) {
   return (int)y;
 }
 
-// LDS view of NF field areas, each holding W interleaved transforms of
-// length L (the W columns of a column block or the RB rows of a row block).
-template <class T, int L, int W, int R>
+// LDS view of NFM field areas, each holding W interleaved transforms of
+// length L (the W columns of a column block or the RB rows of a row block);
+// with PP a second set of NFM areas follows, used by alternate exchanges.
+template <class T, int L, int W, int R, int NFM = 1, int PP = 0>
 struct LdsView {
-  static constexpr int kL = L, kW = W, kR = R;
+  static constexpr int kL = L, kW = W, kR = R, kFields = NFM;
+  static constexpr int kBuf = PP ? NFM * lds_area_elems<L, W, R>() : 0;   // elements
+  static constexpr int kElems = (PP ? 2 : 1) * NFM * lds_area_elems<L, W, R>();
   cx<T>* base;
   int c;
   // unpadded access for the even/odd pair exchange of the real transforms
@@ -186,9 +189,10 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red /* LDS, N
 // ---------------------------------------------------------------------------
 // Row pass.  Block = RB rows (one brick row) x TR = NX2/R lanes per row.
 // G = 3: the three C2R transforms share their LDS exchanges (and the two
-// R2C transforms theirs); G = 1: one field at a time (less LDS).
+// R2C transforms theirs); G = 1: one field at a time (less LDS).  PP: two
+// sets of LDS areas, one barrier per transform exchange (more LDS).
 // ---------------------------------------------------------------------------
-template <class T, int NX2, int RB, int R, int BX, int G>
+template <class T, int NX2, int RB, int R, int BX, int G, int PP>
 __global__ void __launch_bounds__(RB * (NX2 / R))
 row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>* __restrict__ pool) {
   constexpr int TR = NX2 / R;
@@ -196,7 +200,8 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   static_assert(NX2 % BX == 0 && TR % BX == 0, "row lanes must cover whole bricks");
   static_assert(G == 1 || G == 3, "group of 1 or 3 fields");
   constexpr int GR = G == 3 ? 2 : 1;   // R2C group
-  __shared__ cx<T> lds_mem[G * lds_area_elems<NX2, RB, R>()];
+  using View = LdsView<T, NX2, RB, R, G, PP>;
+  __shared__ cx<T> lds_mem[View::kElems];
   __shared__ cx<T> tw_h[TwLayout<NX2, R>::kAlloc];   // the half-length transforms
   __shared__ cx<T> tw_n[NX2];          // exp(-2 pi i k / Nx), k < Nx/2, the even/odd split
   __shared__ double red[4 * ((NT + 63) / 64)];
@@ -209,7 +214,7 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   const int t = threadIdx.x;
   const int yy = t % RB;
   const int j = t / RB;
-  LdsView<T, NX2, RB, R> lds{lds_mem, yy};
+  View lds{lds_mem, yy};
 
   // brick offset of (row tile, position k) for this lane's row (fields hold
   // at most 2^23 elements, so 32-bit offsets from a uniform base suffice)
@@ -366,14 +371,15 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 // kernel keeps only the state it reads live (fewer registers, more waves).  GF = 2 / GI = 3: the forward (P, Q) and inverse
 // (v_x, v_y, omega) transforms share LDS exchanges; 1: one at a time.
 // ---------------------------------------------------------------------------
-template <class T, int NY, int W, int R, int BY, int BX, int GF, int GI, int stage>
+template <class T, int NY, int W, int R, int BY, int BX, int GF, int GI, int PP, int stage>
 __global__ void __launch_bounds__(W * (NY / R))
 col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>* __restrict__ pool) {
   constexpr int TC = NY / R;
   constexpr int NT = W * TC;
   constexpr int GMAX = GF > GI ? GF : GI;
   static_assert((GF == 1 || GF == 2) && (GI == 1 || GI == 3), "field groups");
-  __shared__ cx<T> lds_mem[GMAX * lds_area_elems<NY, W, R>()];
+  using View = LdsView<T, NY, W, R, GMAX, PP>;
+  __shared__ cx<T> lds_mem[View::kElems];
   __shared__ cx<T> tw_y[TwLayout<NY, R>::kAlloc];
   __shared__ double mean_in[7];   // tile 0: sum|u|^2, sum Cx, sum Cy, M, M_old
   __shared__ long long nstep_in;
@@ -389,7 +395,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   const int c = t % W, j = t / W;
   const int x = tile * W + c;
   const int nx2 = d.nx >> 1;
-  LdsView<T, NY, W, R> lds{lds_mem, c};
+  View lds{lds_mem, c};
 
   // bricks of BY x BX; this block's W columns sit inside one brick column
   static_assert(BX % W == 0, "column block within one brick column");

@@ -117,6 +117,8 @@ typedef struct {
   int row_radix, row_group; /* row pass radix, fields per LDS exchange        */
   int col_radix, col_fwd_group, col_inv_group;
   int col_width;            /* columns per column-pass block (divides bx)     */
+  int lds_double;           /* 1: two sets of LDS areas, one barrier per FFT  *
+                             * exchange instead of two (more LDS per block)   */
 } af_schedule;
 int af_schedule_count(int precision, int nx, int ny);
 af_status af_schedule_get(int precision, int nx, int ny, int index, af_schedule* out);

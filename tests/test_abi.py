@@ -80,6 +80,8 @@ def test_tuned_tables_name_instantiated_schedules(af):
             prec, nx, ny, sched = v[0], v[1], v[2], tuple(v[3:])
             if len(sched) == 7:
                 sched = sched + (sched[1],)
+            if len(sched) == 8:
+                sched = sched + (0,)
             assert sched in af.schedules(prec, nx, ny), (name, line)
 
 
@@ -87,5 +89,5 @@ def test_schedule_struct_fields(af):
     """af_schedule carries the column-block width (divides the brick width)."""
     for prec in (32, 64):
         for sc in af.schedules(prec, 1024):
-            by, bx, rr, rg, cr, gf, gi, cw = sc
-            assert bx % cw == 0 and rr in (4, 8, 16) and cr in (4, 8, 16)
+            by, bx, rr, rg, cr, gf, gi, cw, pp = sc
+            assert bx % cw == 0 and rr in (4, 8, 16) and cr in (4, 8, 16) and pp in (0, 1)
