@@ -67,6 +67,9 @@ def candidates(prec, n):
                 (4, 2, 4, 1, 4, 1, 1), (2, 2, 4, 3, 8, 1, 1)]
         narrow = [(4, 2, 4, 3, 4, 2, 3, 1), (4, 2, 8, 3, 8, 2, 3, 1), (4, 2, 8, 1, 8, 1, 1, 1)]
     alts = [a + (a[1], 0) for a in alts] + [a + (0,) for a in narrow]
+    if prec == 32:   # radix-16 rows and columns with 4-row bricks (2048^2, 4096^2)
+        alts += [(4, 2, 16, 1, 16, 1, 1, 2, 0), (4, 2, 16, 1, 16, 1, 1, 1, 0), (2, 2, 16, 1, 16, 1, 1, 2, 0),
+                 (4, 4, 16, 1, 16, 1, 1, 2, 0), (4, 4, 16, 1, 16, 1, 1, 4, 0)]
     # double-buffered LDS exchanges for the single-run schedules of the small sizes
     # (one block per CU there, so the doubled LDS costs no occupancy)
     if prec == 32:

@@ -15,6 +15,7 @@ extern template hipError_t launch_row<float, 128, 2, 4, 2, 3, 0>(const void*, in
 extern template hipError_t launch_row<float, 128, 2, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 128, 2, 8, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 128, 2, 8, 4, 3, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 128, 2, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 128, 4, 4, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 128, 4, 4, 2, 3, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 128, 4, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
@@ -26,11 +27,13 @@ extern template hipError_t launch_row<float, 128, 4, 8, 4, 3, 0>(const void*, in
 extern template hipError_t launch_row<float, 128, 4, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 128, 4, 16, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 128, 4, 16, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 128, 4, 16, 4, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 2, 4, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 2, 4, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 2, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 2, 8, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 2, 8, 4, 3, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 256, 2, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 4, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 4, 2, 3, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
@@ -42,11 +45,13 @@ extern template hipError_t launch_row<float, 256, 4, 8, 4, 3, 0>(const void*, in
 extern template hipError_t launch_row<float, 256, 4, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 16, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 16, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 256, 4, 16, 4, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 2, 4, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 2, 4, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 2, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 2, 8, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 2, 8, 4, 3, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 512, 2, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 4, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 4, 2, 3, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
@@ -58,9 +63,11 @@ extern template hipError_t launch_row<float, 512, 4, 8, 4, 3, 0>(const void*, in
 extern template hipError_t launch_row<float, 512, 4, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 16, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 16, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 512, 4, 16, 4, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 2, 4, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 2, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 2, 8, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 1024, 2, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 8, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 8, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
@@ -69,7 +76,11 @@ extern template hipError_t launch_row<float, 1024, 4, 8, 4, 3, 0>(const void*, i
 extern template hipError_t launch_row<float, 1024, 4, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 16, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 16, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 1024, 4, 16, 4, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 2048, 2, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 2048, 2, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 2048, 4, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 2048, 4, 16, 4, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 32, 2, 8, 2, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 32, 2, 8, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 64, 2, 8, 2, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
@@ -93,7 +104,9 @@ extern template hipError_t launch_col<float, 256, 2, 8, 4, 2, 1, 1, 1>(const voi
 extern template hipError_t launch_col<float, 256, 2, 8, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 2, 8, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 2, 8, 4, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 256, 2, 16, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 2, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 256, 2, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 4, 8, 2, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 4, 8, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 4, 8, 4, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
@@ -115,7 +128,9 @@ extern template hipError_t launch_col<float, 512, 2, 8, 4, 2, 1, 1, 1>(const voi
 extern template hipError_t launch_col<float, 512, 2, 8, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 2, 8, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 2, 8, 4, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 2, 16, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 2, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 2, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 4, 8, 2, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 4, 8, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 4, 8, 4, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
@@ -137,7 +152,9 @@ extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 1, 1, 1>(const vo
 extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 4, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 1024, 2, 16, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 1024, 2, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 4, 8, 2, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 4, 8, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 4, 8, 4, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
@@ -153,12 +170,16 @@ extern template hipError_t launch_col<float, 2048, 2, 8, 4, 2, 1, 1, 0>(const vo
 extern template hipError_t launch_col<float, 2048, 2, 8, 4, 2, 1, 1, 1>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 8, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 8, 4, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 1, 8, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 1, 16, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 4096, 1, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 2, 16, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 2, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 namespace {
 using T = float;
 const RowCfg kRow[] = {
@@ -170,6 +191,7 @@ const RowCfg kRow[] = {
     {128, 2, 8, 2, 32, launch_row<T, 128, 2, 8, 2, 1, 0>, 1, 0},
     {128, 2, 8, 2, 32, launch_row<T, 128, 2, 8, 2, 3, 0>, 3, 0},
     {128, 2, 8, 4, 32, launch_row<T, 128, 2, 8, 4, 3, 0>, 3, 0},
+    {128, 2, 16, 2, 16, launch_row<T, 128, 2, 16, 2, 1, 0>, 1, 0},
     {128, 4, 4, 2, 128, launch_row<T, 128, 4, 4, 2, 3, 0>, 3, 0},
     {128, 4, 4, 2, 128, launch_row<T, 128, 4, 4, 2, 3, 1>, 3, 1},
     {128, 4, 8, 2, 64, launch_row<T, 128, 4, 8, 2, 1, 0>, 1, 0},
@@ -181,11 +203,13 @@ const RowCfg kRow[] = {
     {128, 4, 16, 2, 32, launch_row<T, 128, 4, 16, 2, 1, 0>, 1, 0},
     {128, 4, 16, 2, 32, launch_row<T, 128, 4, 16, 2, 1, 1>, 1, 1},
     {128, 4, 16, 2, 32, launch_row<T, 128, 4, 16, 2, 3, 0>, 3, 0},
+    {128, 4, 16, 4, 32, launch_row<T, 128, 4, 16, 4, 1, 0>, 1, 0},
     {256, 2, 4, 2, 128, launch_row<T, 256, 2, 4, 2, 1, 0>, 1, 0},
     {256, 2, 4, 2, 128, launch_row<T, 256, 2, 4, 2, 3, 0>, 3, 0},
     {256, 2, 8, 2, 64, launch_row<T, 256, 2, 8, 2, 1, 0>, 1, 0},
     {256, 2, 8, 2, 64, launch_row<T, 256, 2, 8, 2, 3, 0>, 3, 0},
     {256, 2, 8, 4, 64, launch_row<T, 256, 2, 8, 4, 3, 0>, 3, 0},
+    {256, 2, 16, 2, 32, launch_row<T, 256, 2, 16, 2, 1, 0>, 1, 0},
     {256, 4, 4, 2, 256, launch_row<T, 256, 4, 4, 2, 3, 0>, 3, 0},
     {256, 4, 4, 2, 256, launch_row<T, 256, 4, 4, 2, 3, 1>, 3, 1},
     {256, 4, 8, 2, 128, launch_row<T, 256, 4, 8, 2, 1, 0>, 1, 0},
@@ -197,11 +221,13 @@ const RowCfg kRow[] = {
     {256, 4, 16, 2, 64, launch_row<T, 256, 4, 16, 2, 1, 0>, 1, 0},
     {256, 4, 16, 2, 64, launch_row<T, 256, 4, 16, 2, 1, 1>, 1, 1},
     {256, 4, 16, 2, 64, launch_row<T, 256, 4, 16, 2, 3, 0>, 3, 0},
+    {256, 4, 16, 4, 64, launch_row<T, 256, 4, 16, 4, 1, 0>, 1, 0},
     {512, 2, 4, 2, 256, launch_row<T, 512, 2, 4, 2, 1, 0>, 1, 0},
     {512, 2, 4, 2, 256, launch_row<T, 512, 2, 4, 2, 3, 0>, 3, 0},
     {512, 2, 8, 2, 128, launch_row<T, 512, 2, 8, 2, 1, 0>, 1, 0},
     {512, 2, 8, 2, 128, launch_row<T, 512, 2, 8, 2, 3, 0>, 3, 0},
     {512, 2, 8, 4, 128, launch_row<T, 512, 2, 8, 4, 3, 0>, 3, 0},
+    {512, 2, 16, 2, 64, launch_row<T, 512, 2, 16, 2, 1, 0>, 1, 0},
     {512, 4, 4, 2, 512, launch_row<T, 512, 4, 4, 2, 3, 0>, 3, 0},
     {512, 4, 4, 2, 512, launch_row<T, 512, 4, 4, 2, 3, 1>, 3, 1},
     {512, 4, 8, 2, 256, launch_row<T, 512, 4, 8, 2, 1, 0>, 1, 0},
@@ -213,9 +239,11 @@ const RowCfg kRow[] = {
     {512, 4, 16, 2, 128, launch_row<T, 512, 4, 16, 2, 1, 0>, 1, 0},
     {512, 4, 16, 2, 128, launch_row<T, 512, 4, 16, 2, 1, 1>, 1, 1},
     {512, 4, 16, 2, 128, launch_row<T, 512, 4, 16, 2, 3, 0>, 3, 0},
+    {512, 4, 16, 4, 128, launch_row<T, 512, 4, 16, 4, 1, 0>, 1, 0},
     {1024, 2, 4, 2, 512, launch_row<T, 1024, 2, 4, 2, 3, 0>, 3, 0},
     {1024, 2, 8, 2, 256, launch_row<T, 1024, 2, 8, 2, 1, 0>, 1, 0},
     {1024, 2, 8, 2, 256, launch_row<T, 1024, 2, 8, 2, 3, 0>, 3, 0},
+    {1024, 2, 16, 2, 128, launch_row<T, 1024, 2, 16, 2, 1, 0>, 1, 0},
     {1024, 4, 8, 2, 512, launch_row<T, 1024, 4, 8, 2, 1, 0>, 1, 0},
     {1024, 4, 8, 2, 512, launch_row<T, 1024, 4, 8, 2, 1, 1>, 1, 1},
     {1024, 4, 8, 2, 512, launch_row<T, 1024, 4, 8, 2, 3, 0>, 3, 0},
@@ -224,7 +252,11 @@ const RowCfg kRow[] = {
     {1024, 4, 16, 2, 256, launch_row<T, 1024, 4, 16, 2, 1, 0>, 1, 0},
     {1024, 4, 16, 2, 256, launch_row<T, 1024, 4, 16, 2, 1, 1>, 1, 1},
     {1024, 4, 16, 2, 256, launch_row<T, 1024, 4, 16, 2, 3, 0>, 3, 0},
+    {1024, 4, 16, 4, 256, launch_row<T, 1024, 4, 16, 4, 1, 0>, 1, 0},
     {2048, 2, 8, 2, 512, launch_row<T, 2048, 2, 8, 2, 1, 0>, 1, 0},
+    {2048, 2, 16, 2, 256, launch_row<T, 2048, 2, 16, 2, 1, 0>, 1, 0},
+    {2048, 4, 16, 2, 512, launch_row<T, 2048, 4, 16, 2, 1, 0>, 1, 0},
+    {2048, 4, 16, 4, 512, launch_row<T, 2048, 4, 16, 4, 1, 0>, 1, 0},
 };
 const ColCfg kCol[] = {
     {32, 2, 8, 2, 2, 8, launch_col<T, 32, 2, 8, 2, 2, 2, 3, 0>, 2, 3, 0},
@@ -250,7 +282,9 @@ const ColCfg kCol[] = {
     {256, 2, 8, 4, 2, 64, launch_col<T, 256, 2, 8, 4, 2, 2, 3, 0>, 2, 3, 0},
     {256, 2, 8, 4, 2, 64, launch_col<T, 256, 2, 8, 4, 2, 2, 3, 1>, 2, 3, 1},
     {256, 2, 8, 4, 4, 64, launch_col<T, 256, 2, 8, 4, 4, 2, 3, 0>, 2, 3, 0},
+    {256, 2, 16, 2, 2, 32, launch_col<T, 256, 2, 16, 2, 2, 1, 1, 0>, 1, 1, 0},
     {256, 2, 16, 4, 2, 32, launch_col<T, 256, 2, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
+    {256, 2, 16, 4, 4, 32, launch_col<T, 256, 2, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {256, 4, 8, 2, 4, 128, launch_col<T, 256, 4, 8, 2, 4, 2, 3, 0>, 2, 3, 0},
     {256, 4, 8, 4, 4, 128, launch_col<T, 256, 4, 8, 4, 4, 1, 1, 0>, 1, 1, 0},
     {256, 4, 8, 4, 4, 128, launch_col<T, 256, 4, 8, 4, 4, 2, 3, 0>, 2, 3, 0},
@@ -272,7 +306,9 @@ const ColCfg kCol[] = {
     {512, 2, 8, 4, 2, 128, launch_col<T, 512, 2, 8, 4, 2, 2, 3, 0>, 2, 3, 0},
     {512, 2, 8, 4, 2, 128, launch_col<T, 512, 2, 8, 4, 2, 2, 3, 1>, 2, 3, 1},
     {512, 2, 8, 4, 4, 128, launch_col<T, 512, 2, 8, 4, 4, 2, 3, 0>, 2, 3, 0},
+    {512, 2, 16, 2, 2, 64, launch_col<T, 512, 2, 16, 2, 2, 1, 1, 0>, 1, 1, 0},
     {512, 2, 16, 4, 2, 64, launch_col<T, 512, 2, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
+    {512, 2, 16, 4, 4, 64, launch_col<T, 512, 2, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {512, 4, 8, 2, 4, 256, launch_col<T, 512, 4, 8, 2, 4, 2, 3, 0>, 2, 3, 0},
     {512, 4, 8, 4, 4, 256, launch_col<T, 512, 4, 8, 4, 4, 1, 1, 0>, 1, 1, 0},
     {512, 4, 8, 4, 4, 256, launch_col<T, 512, 4, 8, 4, 4, 2, 3, 0>, 2, 3, 0},
@@ -294,7 +330,9 @@ const ColCfg kCol[] = {
     {1024, 2, 8, 4, 2, 256, launch_col<T, 1024, 2, 8, 4, 2, 2, 3, 0>, 2, 3, 0},
     {1024, 2, 8, 4, 2, 256, launch_col<T, 1024, 2, 8, 4, 2, 2, 3, 1>, 2, 3, 1},
     {1024, 2, 8, 4, 4, 256, launch_col<T, 1024, 2, 8, 4, 4, 2, 3, 0>, 2, 3, 0},
+    {1024, 2, 16, 2, 2, 128, launch_col<T, 1024, 2, 16, 2, 2, 1, 1, 0>, 1, 1, 0},
     {1024, 2, 16, 4, 2, 128, launch_col<T, 1024, 2, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
+    {1024, 2, 16, 4, 4, 128, launch_col<T, 1024, 2, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {1024, 4, 8, 2, 4, 512, launch_col<T, 1024, 4, 8, 2, 4, 2, 3, 0>, 2, 3, 0},
     {1024, 4, 8, 4, 4, 512, launch_col<T, 1024, 4, 8, 4, 4, 1, 1, 0>, 1, 1, 0},
     {1024, 4, 8, 4, 4, 512, launch_col<T, 1024, 4, 8, 4, 4, 2, 3, 0>, 2, 3, 0},
@@ -310,12 +348,16 @@ const ColCfg kCol[] = {
     {2048, 2, 8, 4, 2, 512, launch_col<T, 2048, 2, 8, 4, 2, 1, 1, 1>, 1, 1, 1},
     {2048, 2, 8, 4, 2, 512, launch_col<T, 2048, 2, 8, 4, 2, 2, 3, 0>, 2, 3, 0},
     {2048, 2, 8, 4, 4, 512, launch_col<T, 2048, 2, 8, 4, 4, 2, 3, 0>, 2, 3, 0},
+    {2048, 2, 16, 2, 2, 256, launch_col<T, 2048, 2, 16, 2, 2, 1, 1, 0>, 1, 1, 0},
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
+    {2048, 2, 16, 4, 4, 256, launch_col<T, 2048, 2, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {4096, 1, 8, 2, 2, 512, launch_col<T, 4096, 1, 8, 2, 2, 1, 1, 0>, 1, 1, 0},
     {4096, 1, 16, 2, 2, 256, launch_col<T, 4096, 1, 16, 2, 2, 1, 1, 0>, 1, 1, 0},
+    {4096, 1, 16, 4, 2, 256, launch_col<T, 4096, 1, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
     {4096, 2, 16, 2, 2, 512, launch_col<T, 4096, 2, 16, 2, 2, 1, 1, 0>, 1, 1, 0},
     {4096, 2, 16, 4, 2, 512, launch_col<T, 4096, 2, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
+    {4096, 2, 16, 4, 4, 512, launch_col<T, 4096, 2, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
 };
 const Schedule kSched[] = {
     {16, 32, 4, 2, 4, 3, 8, 2, 3, 2, 0, 1},
@@ -373,6 +415,11 @@ const Schedule kSched[] = {
     {128, 256, 4, 2, 16, 1, 8, 1, 1, 2, 0, 0},
     {128, 256, 4, 2, 16, 3, 8, 1, 1, 2, 0, 0},
     {128, 256, 4, 2, 16, 1, 4, 2, 3, 2, 0, 0},
+    {128, 256, 4, 2, 16, 1, 16, 1, 1, 2, 0, 0},
+    {128, 256, 4, 2, 16, 1, 16, 1, 1, 1, 0, 0},
+    {128, 256, 2, 2, 16, 1, 16, 1, 1, 2, 0, 0},
+    {128, 256, 4, 4, 16, 1, 16, 1, 1, 2, 0, 0},
+    {128, 256, 4, 4, 16, 1, 16, 1, 1, 4, 0, 0},
     {128, 256, 4, 2, 4, 3, 4, 2, 3, 2, 1, 0},
     {128, 256, 4, 2, 4, 3, 4, 2, 3, 1, 1, 0},
     {128, 256, 4, 2, 8, 3, 8, 2, 3, 2, 1, 0},
@@ -414,6 +461,11 @@ const Schedule kSched[] = {
     {256, 512, 4, 2, 16, 1, 8, 1, 1, 2, 0, 0},
     {256, 512, 4, 2, 16, 3, 8, 1, 1, 2, 0, 0},
     {256, 512, 4, 2, 16, 1, 4, 2, 3, 2, 0, 0},
+    {256, 512, 4, 2, 16, 1, 16, 1, 1, 2, 0, 0},
+    {256, 512, 4, 2, 16, 1, 16, 1, 1, 1, 0, 0},
+    {256, 512, 2, 2, 16, 1, 16, 1, 1, 2, 0, 0},
+    {256, 512, 4, 4, 16, 1, 16, 1, 1, 2, 0, 0},
+    {256, 512, 4, 4, 16, 1, 16, 1, 1, 4, 0, 0},
     {256, 512, 4, 2, 4, 3, 4, 2, 3, 2, 1, 0},
     {256, 512, 4, 2, 4, 3, 4, 2, 3, 1, 1, 0},
     {256, 512, 4, 2, 8, 3, 8, 2, 3, 2, 1, 0},
@@ -455,6 +507,11 @@ const Schedule kSched[] = {
     {512, 1024, 4, 2, 16, 1, 8, 1, 1, 2, 0, 0},
     {512, 1024, 4, 2, 16, 3, 8, 1, 1, 2, 0, 0},
     {512, 1024, 4, 2, 16, 1, 4, 2, 3, 2, 0, 0},
+    {512, 1024, 4, 2, 16, 1, 16, 1, 1, 2, 0, 0},
+    {512, 1024, 4, 2, 16, 1, 16, 1, 1, 1, 0, 0},
+    {512, 1024, 2, 2, 16, 1, 16, 1, 1, 2, 0, 0},
+    {512, 1024, 4, 4, 16, 1, 16, 1, 1, 2, 0, 0},
+    {512, 1024, 4, 4, 16, 1, 16, 1, 1, 4, 0, 0},
     {512, 1024, 4, 2, 4, 3, 4, 2, 3, 2, 1, 0},
     {512, 1024, 4, 2, 4, 3, 4, 2, 3, 1, 1, 0},
     {512, 1024, 4, 2, 8, 3, 8, 2, 3, 2, 1, 0},
@@ -484,6 +541,11 @@ const Schedule kSched[] = {
     {1024, 2048, 4, 4, 8, 3, 16, 1, 1, 4, 0, 0},
     {1024, 2048, 4, 2, 16, 1, 8, 1, 1, 2, 0, 0},
     {1024, 2048, 4, 2, 16, 3, 8, 1, 1, 2, 0, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 0, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 1, 0, 0},
+    {1024, 2048, 2, 2, 16, 1, 16, 1, 1, 2, 0, 0},
+    {1024, 2048, 4, 4, 16, 1, 16, 1, 1, 2, 0, 0},
+    {1024, 2048, 4, 4, 16, 1, 16, 1, 1, 4, 0, 0},
     {1024, 2048, 4, 2, 16, 1, 8, 1, 1, 2, 1, 0},
     {1024, 2048, 4, 2, 8, 1, 8, 1, 1, 2, 1, 0},
     {1024, 4096, 4, 2, 8, 3, 16, 1, 1, 2, 0, 1},
@@ -497,6 +559,10 @@ const Schedule kSched[] = {
     {2048, 4096, 2, 2, 8, 1, 16, 1, 1, 2, 0, 1},
     {2048, 4096, 2, 2, 8, 1, 16, 1, 1, 1, 0, 0},
     {2048, 4096, 2, 2, 8, 1, 8, 1, 1, 1, 0, 0},
+    {2048, 4096, 4, 2, 16, 1, 16, 1, 1, 2, 0, 0},
+    {2048, 4096, 4, 2, 16, 1, 16, 1, 1, 1, 0, 0},
+    {2048, 4096, 2, 2, 16, 1, 16, 1, 1, 2, 0, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 0, 0},
 };
 }  // namespace
 
