@@ -35,6 +35,13 @@ def field_bytes(n, prec):
     return n * (n // 2) * (8 if prec == 32 else 16)
 
 
+def step_model_bytes(n, prec=32):
+    """Bytes per step of the 3-pass model of one IF-RK step: 76 transfers of an
+    Ny x (Nx/2+1) complex field (what a schedule of separate FFT, pointwise and
+    update passes moves; this design moves 50 transfers of Ny x Nx/2)."""
+    return 76 * n * (n // 2 + 1) * (8 if prec == 32 else 16)
+
+
 # Compulsory HBM traffic per launch in complex-field transfers (read + write):
 #   row pass: reads v_x, v_y, omega (mixed), writes P, Q             -> 5
 #   column pass: reads P, Q, omega_old (stages 1-3), acc (stages 2-4);
@@ -131,7 +138,9 @@ def ensemble_leg(af, steps=200, seed=7):
             "value": value, "ms_per_step": wall / steps * 1e3, "sequential_value": pts * steps / seq,
             "speedup_vs_sequential": seq / wall,
             "step_bytes_50_transfers": 50 * fb,
-            "hbm_frac_50_transfers": 50 * fb / (wall / steps) / 1e9 / HBM_PEAK_GBS}
+            "hbm_frac_50_transfers": 50 * fb / (wall / steps) / 1e9 / HBM_PEAK_GBS,
+            "step_model_frac": sum(cnt * step_model_bytes(n) for n, cnt in mix) / (wall / steps) / 1e9
+            / HBM_PEAK_GBS}
 
 
 def batched_leg(af, n, count, steps, seed=3):
@@ -151,7 +160,8 @@ def batched_leg(af, n, count, steps, seed=3):
     fb = field_bytes(n, 32) * count
     return {"grid": [n, n], "instances": count, "steps": steps, "value": count * n * n * steps / wall,
             "ms_per_step": wall / steps * 1e3,
-            "hbm_frac_50_transfers": 50 * fb / (wall / steps) / 1e9 / HBM_PEAK_GBS}
+            "hbm_frac_50_transfers": 50 * fb / (wall / steps) / 1e9 / HBM_PEAK_GBS,
+            "step_model_frac": count * step_model_bytes(n) / (wall / steps) / 1e9 / HBM_PEAK_GBS}
 
 
 def serving_leg(af, rate, window=0.5, seed=11, horizon_scale=0.1):
@@ -236,7 +246,7 @@ def main():
             extra[f"{n}_fp{prec}"] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "dominant": nm,
                                       "dominant_ms": ms, "achieved_GBs": ach,
                                       "frac": ach / HBM_PEAK_GBS,
-                                      "step_model_frac": 76 * n * (n // 2 + 1) * (8 if prec == 32 else 16)
+                                      "step_model_frac": step_model_bytes(n, prec)
                                       / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
         extra["fp32_over_fp64_1024"] = head["value"] / extra["1024_fp64"]["value"]
     out = {
@@ -260,7 +270,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": tr_bytes,
                      "algorithmic_bytes_per_launch": alg, "avg_launch_ms": kms,
                      "events_over_steps": head["prof_steps"],
-                     "step_model_frac": 76 * 1024 * 513 * 8 / (head["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                     "step_model_frac": step_model_bytes(1024) / (head["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "extra": extra,
     }
     if not args.no_ensemble:

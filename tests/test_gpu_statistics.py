@@ -2,6 +2,8 @@
 20k steps): per-step relative L2 of omega for the first 200 steps, and the
 time averages of energy, enstrophy and the shell spectrum over steps
 10k-20k with batch-means error bars (10 batches)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -101,3 +103,71 @@ def test_long_run_fp32_vs_fp64_statistics(af):
         sig = np.hypot(dS32, dS64)
         band = S64 > 1e-3 * S64.max()
         assert np.all(np.abs(S32 - S64)[band] <= 4 * sig[band] + 0.02 * S64[band])
+
+
+@pytest.mark.skipif(os.environ.get("AF_LONG_STATS") != "1",
+                    reason="long run (about 3 minutes of GPU time): set AF_LONG_STATS=1")
+@pytest.mark.timeout(900)
+def test_long_run_energy_within_one_percent(af):
+    """Stationary window: 1024^2, dt = 1e-4, two seeds, 300k steps (t = 30),
+    averaged over steps 20k-300k.  FP32 runs go through the ensemble engine
+    (the batched form, both seeds in one launch per pass); FP64 runs are
+    plain single runs, the GPU path that equals the oracle to 1e-12 per step.
+    Energies come from the per-step energy series (means of 100 steps),
+    enstrophy and E(k) from the diagnostics every 1000 steps; error bars are
+    batch means (10 batches per seed) combined over the two seeds."""
+    N, dt, total, start, chunk = 1024, 1e-4, 300000, 20000, 1000
+    seeds = (1, 2)
+    eng = af.Engine(8 << 30)
+    slots = [eng.add(af.params(N, dt=dt, precision=32, **JW), af.ic_bandlimited(N, seed=s, amp=1e-3, kmax=10.0))
+             for s in seeds]
+    runs = [af.Run(af.params(N, dt=dt, precision=64, **JW)) for _ in seeds]
+    for r, s in zip(runs, seeds):
+        r.set_state(af.ic_bandlimited(N, seed=s, amp=1e-3, kmax=10.0))
+    e32 = [[] for _ in seeds]
+    e64 = [[] for _ in seeds]
+    d32 = [[] for _ in seeds]
+    d64 = [[] for _ in seeds]
+    done = 0
+    while done < total:
+        eng.step(chunk)
+        for i, r in enumerate(runs):
+            e64[i].append(r.step(chunk, energy=True).reshape(-1, 100).mean(axis=1))
+            e32[i].append(eng.energy(slots[i], chunk).reshape(-1, 100).mean(axis=1))
+        done += chunk
+        if done > start:
+            for i, r in enumerate(runs):
+                d32[i].append(eng.diagnostics(slots[i], nshell=64))
+                d64[i].append(r.diagnostics(nshell=64))
+        if done % 50000 == 0:
+            print(f"step {done}", flush=True)
+    eng.close()
+    for r in runs:
+        r.close()
+
+    def combine(per_seed):
+        m = np.mean([p[0] for p in per_seed], axis=0)
+        s = np.sqrt(np.sum([p[1] ** 2 for p in per_seed], axis=0)) / len(per_seed)
+        return m, s
+
+    k0 = start // 100
+    E32, dE32 = combine([_batch_means(np.concatenate(e)[k0:]) for e in e32])
+    E64, dE64 = combine([_batch_means(np.concatenate(e)[k0:]) for e in e64])
+    Z32, dZ32 = combine([_batch_means([x[1] for x in d]) for d in d32])
+    Z64, dZ64 = combine([_batch_means([x[1] for x in d]) for d in d64])
+    S32, dS32 = combine([_batch_means(np.array([x[2] for x in d])) for d in d32])
+    S64, dS64 = combine([_batch_means(np.array([x[2] for x in d])) for d in d64])
+    rel = abs(E32 - E64) / E64
+    print(f"\nsteps {start}-{total}, seeds {seeds}: <E> fp32 (engine) {E32:.6g} +- {dE32:.2g}  "
+          f"fp64 {E64:.6g} +- {dE64:.2g}  rel diff {rel:.2e} ({abs(E32 - E64) / np.hypot(dE32, dE64):.2f} sigma)")
+    print(f"<Z> fp32 {Z32:.6g} +- {dZ32:.2g}  fp64 {Z64:.6g} +- {dZ64:.2g}  "
+          f"rel diff {abs(Z32 - Z64) / Z64:.2e}")
+    band = S64 > 1e-3 * S64.max()
+    sig = np.hypot(dS32, dS64)
+    print("E(k) shells 1-8 fp32 / fp64 / sigma:")
+    for k in range(1, 9):
+        print(f"  k={k}: {S32[k]:.5g} {S64[k]:.5g} {sig[k]:.2g}")
+    assert rel <= 0.01, rel
+    assert abs(E32 - E64) <= 4 * np.hypot(dE32, dE64)
+    assert abs(Z32 - Z64) <= 4 * np.hypot(dZ32, dZ64)
+    assert np.all(np.abs(S32 - S64)[band] <= 4 * sig[band] + 0.02 * S64[band])
