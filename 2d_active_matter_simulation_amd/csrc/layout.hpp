@@ -25,8 +25,8 @@ struct InstDesc {
   cx<T>* um;    // mixed v_x
   cx<T>* vm;    // mixed v_y
   cx<T>* wm;    // mixed omega
-  cx<T>* pm;    // mixed P = -i kx (lam A + beta Cy)^x
-  cx<T>* qm;    // mixed Q = (lam W - beta Cx)^x
+  cx<T>* pm;    // mixed 2P, P = -i kx (lam A + beta Cy)^x
+  cx<T>* qm;    // mixed 2Q, Q = (lam W - beta Cx)^x  (the 1/2 is applied by the column pass)
   float* ifac;  // FP32 runs: integrating factors in column-block order, plane
                 // 0 exp(sigma dt/2), plane 1 exp(sigma dt) (FP64-evaluated,
                 // rounded once); null for FP64 runs (evaluated in the kernel)

@@ -10,7 +10,7 @@
 //     P = -i kx F1^ ; Q = F2^      (the kx half of nonl_subfunc_2, .cu:1221)
 //     FP64 block partials of sum|u|^2 (energy, .cu:1118-1127) and sum C
 //     (mean-velocity RHS, .cu:1086-1087)
-//     writes P, Q
+//     writes 2P, 2Q (the column pass folds the 1/2 into the mask factor)
 //
 //   column pass (one block per brick column)
 //     reads  P, Q, omega_old, acc
@@ -307,15 +307,16 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   }
 
   // Row R2C of packed real pairs: Z = FFT_M(z), then
-  // X[k] = 1/2 (Z[k] + conj Z[M-k]) - 1/2 i e^{-2 pi i k/N} (Z[k] - conj Z[M-k]).
+  // 2 X[k] = (Z[k] + conj Z[M-k]) - i e^{-2 pi i k/N} (Z[k] - conj Z[M-k]).
+  // P and Q are stored as twice the half spectra: the column pass applies
+  // the 1/2 together with the dealiasing mask (an exact power-of-two scaling
+  // that commutes with the transforms, so the results are unchanged).
   auto r2c_post = [&](cx<T>(&q)[R], int f) {
-    const T h = (T)0.5;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int k = j + r * TR;
       const cx<T> zk = q[r], zm = cconj(lds.load_plain(f, (NX2 - k) & (NX2 - 1)));
-      const cx<T> a = cscale(h, zk + zm), dd = zk - zm;
-      q[r] = a + cmul(cmuli(cscale(-h, dd)), tw_n[k]);
+      q[r] = (zk + zm) - cmul(cmuli(zk - zm), tw_n[k]);
     }
   };
   if constexpr (GR == 2) {
@@ -546,7 +547,8 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
       const int nys = (y <= NY / 2) ? y : y - NY;
       const double kyd = d.tky * (double)nys;
       const double k2 = kx2 + kyd * kyd;
-      const T keepf = ((nys < 0 ? -nys : nys) <= ymax) ? (T)1 : (T)0;
+      // the row pass stores 2 P, 2 Q: the mask factor carries the 1/2
+      const T keepf = ((nys < 0 ? -nys : nys) <= ymax) ? (T)0.5 : (T)0;
       // N^ = (P^ - i ky Q^) * mask
       const cx<T> nh = cscale(keepf, pq[0][r] + cmuli(cscale((T)(-kyd), pq[1][r])));
       T E = (T)1, E2 = (T)1;
