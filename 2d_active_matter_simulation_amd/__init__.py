@@ -42,11 +42,11 @@ class AfParams(ctypes.Structure):
 class AfSchedule(ctypes.Structure):
     _fields_ = [("by", ctypes.c_int), ("bx", ctypes.c_int), ("row_radix", ctypes.c_int),
                 ("row_group", ctypes.c_int), ("col_radix", ctypes.c_int), ("col_fwd_group", ctypes.c_int),
-                ("col_inv_group", ctypes.c_int), ("col_width", ctypes.c_int), ("lds_double", ctypes.c_int)]
+                ("col_inv_group", ctypes.c_int), ("col_width", ctypes.c_int), ("options", ctypes.c_int)]
 
     def as_tuple(self):
         return (self.by, self.bx, self.row_radix, self.row_group, self.col_radix, self.col_fwd_group,
-                self.col_inv_group, self.col_width, self.lds_double)
+                self.col_inv_group, self.col_width, self.options)
 
 
 class AfRequest(ctypes.Structure):

@@ -73,7 +73,7 @@ def main():
     args = ap.parse_args()
     if args.out is None:
         args.out = os.path.join(HERE, "schedules_batched.txt" if args.instances > 1 else "schedules.txt")
-    lines = ["# prec nx ny by bx row_radix row_group col_radix col_fwd_group col_inv_group col_width lds_double  (ms/step)"]
+    lines = ["# prec nx ny by bx row_radix row_group col_radix col_fwd_group col_inv_group col_width options  (ms/step)"]
     for prec in args.precisions:
         for n in args.sizes:
             scheds = af.schedules(prec, n)
@@ -94,6 +94,13 @@ def main():
             if best:
                 lines.append(f"{prec} {n} {n} " + " ".join(map(str, best[1])) + f"   # {best[0]:.4f}")
                 print(f"  -> best {best[1]} {best[0]:.4f} ms/step", flush=True)
+    # keep the existing table's lines for the (precision, size) pairs not tuned now
+    tuned = {tuple(ln.split()[:3]) for ln in lines[1:]}
+    if os.path.exists(args.out):
+        for ln in open(args.out):
+            key = tuple(ln.split()[:3])
+            if ln.strip() and not ln.startswith("#") and key not in tuned:
+                lines.append(ln.rstrip("\n"))
     text = "\n".join(lines) + "\n"
     if args.dry:
         print(text)

@@ -25,11 +25,26 @@ def lds_area(L, W, R):
 
 
 def lds_row(nx2, rb, R, G, esz, pp=0):
-    return ((2 if pp else 1) * G * lds_area(nx2, rb, R) + 2 * nx2) * esz
+    return ((2 if pp & 1 else 1) * G * lds_area(nx2, rb, R) + 2 * nx2) * esz
 
 
 def lds_col(ny, W, R, GF, GI, esz, pp=0):
-    return ((2 if pp else 1) * max(GF, GI) * lds_area(ny, W, R) + ny) * esz
+    return ((2 if pp & 1 else 1) * max(GF, GI) * lds_area(ny, W, R) + ny) * esz
+
+
+# Option bits of a schedule (its last field, the kernels' PP argument):
+# 1 double-buffered LDS exchanges, 2 streaming column-pass stores, 4 streaming
+# row-pass stores.  The row kernel reads bits 1 and 4, the column kernel 1 and 2.
+ROW_OPTS, COL_OPTS = 5, 3
+# FP32 schedules that also get the streaming-store variants (options | 2, 4,
+# 6): the fastest single-run and batched schedules of each size.
+STORE_VARIANT_BASES = {
+    256: [(4, 2, 4, 3, 4, 2, 3, 2, 1)],
+    512: [(4, 2, 4, 3, 4, 2, 3, 1, 0), (4, 4, 8, 3, 8, 2, 3, 4, 0)],
+    1024: [(4, 2, 4, 3, 4, 2, 3, 2, 1), (4, 2, 8, 1, 8, 1, 1, 2, 0)],
+    2048: [(4, 2, 16, 1, 16, 1, 1, 2, 0), (4, 4, 8, 1, 16, 1, 1, 4, 0)],
+    4096: [(4, 4, 16, 1, 16, 1, 1, 2, 0)],
+}
 
 
 def default_schedule(prec, nx, ny):
@@ -77,6 +92,8 @@ def candidates(prec, n):
                  (4, 2, 16, 1, 8, 1, 1, 2, 1), (4, 2, 8, 1, 8, 1, 1, 2, 1)]
     else:
         alts += [(4, 2, 4, 3, 4, 2, 3, 2, 1), (4, 1, 8, 3, 8, 2, 3, 1, 1), (4, 2, 8, 1, 8, 1, 1, 1, 1)]
+    if prec == 32:
+        alts += [b[:8] + (b[8] | o,) for b in STORE_VARIANT_BASES.get(n, []) for o in (2, 4, 6)]
     for s in alts:
         if s not in out and feasible(prec, n, n, s):
             out.append(s)
@@ -105,6 +122,7 @@ HDR = """// {what} schedule and launch tables (the kernels are instantiated in
 // Row kernel:    (Nx/2, RB = brick height, R, BX = brick width, G, PP)
 // Column kernel: (Ny, W = columns per block, R, BY, BX = brick height/width, GF, GI, PP)
 // Schedule:      (Nx/2, Ny, BY, BX, row R, row G, col R, col GF, col GI, col W, PP, default)
+// PP: option bits (1 double-buffered LDS, 2 / 4 streaming column / row stores)
 #include "kernels_inst.hpp"
 
 namespace af {{
@@ -146,8 +164,8 @@ def gen(prec):
                     scheds.append((nx // 2, ny) + s + (0,))
     rows, cols = set(), set()
     for (nx2, ny, by, bx, rr, rg, cr, gf, gi, cw, pp, _) in scheds:
-        rows.add((nx2, by, rr, bx, rg, pp))
-        cols.add((ny, cw, cr, by, bx, gf, gi, pp))
+        rows.add((nx2, by, rr, bx, rg, pp & ROW_OPTS))
+        cols.add((ny, cw, cr, by, bx, gf, gi, pp & COL_OPTS))
     rl = [f"    {{{a}, {b}, {r}, {c}, {b * (a // r)}, launch_row<T, {a}, {b}, {r}, {c}, {g}, {pp}>, {g}, {pp}}},"
           for (a, b, r, c, g, pp) in sorted(rows)]
     cl = [f"    {{{a}, {w}, {r}, {by}, {bx}, {w * (a // r)}, launch_col<T, {a}, {w}, {r}, {by}, {bx}, {gf}, {gi}, {pp}>, "

@@ -4,6 +4,7 @@
 // Row kernel:    (Nx/2, RB = brick height, R, BX = brick width, G, PP)
 // Column kernel: (Ny, W = columns per block, R, BY, BX = brick height/width, GF, GI, PP)
 // Schedule:      (Nx/2, Ny, BY, BX, row R, row G, col R, col GF, col GI, col W, PP, default)
+// PP: option bits (1 double-buffered LDS, 2 / 4 streaming column / row stores)
 #include "kernels_inst.hpp"
 
 namespace af {
@@ -18,6 +19,7 @@ extern template hipError_t launch_row<float, 128, 2, 8, 4, 3, 0>(const void*, in
 extern template hipError_t launch_row<float, 128, 2, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 128, 4, 4, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 128, 4, 4, 2, 3, 1>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 128, 4, 4, 2, 3, 5>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 128, 4, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 128, 4, 8, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 128, 4, 8, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
@@ -36,12 +38,14 @@ extern template hipError_t launch_row<float, 256, 2, 8, 4, 3, 0>(const void*, in
 extern template hipError_t launch_row<float, 256, 2, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 4, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 4, 2, 3, 1>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 256, 4, 4, 2, 3, 4>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 8, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 8, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 8, 2, 3, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 8, 4, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 8, 4, 3, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 256, 4, 8, 4, 3, 4>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 16, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 16, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
@@ -54,8 +58,10 @@ extern template hipError_t launch_row<float, 512, 2, 8, 4, 3, 0>(const void*, in
 extern template hipError_t launch_row<float, 512, 2, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 4, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 4, 2, 3, 1>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 512, 4, 4, 2, 3, 5>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 8, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 512, 4, 8, 2, 1, 4>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 8, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 8, 2, 3, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 8, 4, 1, 0>(const void*, int, int, const void*, hipStream_t);
@@ -72,15 +78,18 @@ extern template hipError_t launch_row<float, 1024, 4, 8, 2, 1, 0>(const void*, i
 extern template hipError_t launch_row<float, 1024, 4, 8, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 8, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 8, 4, 1, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 1024, 4, 8, 4, 1, 4>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 8, 4, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 16, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 1024, 4, 16, 2, 1, 4>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 16, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 16, 4, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 2048, 2, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 2048, 2, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 2048, 4, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 2048, 4, 16, 4, 1, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 2048, 4, 16, 4, 1, 4>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 32, 2, 8, 2, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 32, 2, 8, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 64, 2, 8, 2, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
@@ -97,6 +106,7 @@ extern template hipError_t launch_col<float, 256, 1, 16, 4, 2, 1, 1, 0>(const vo
 extern template hipError_t launch_col<float, 256, 2, 4, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 2, 4, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 2, 4, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 256, 2, 4, 4, 2, 2, 3, 3>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 2, 8, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 2, 8, 2, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 2, 8, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
@@ -113,6 +123,7 @@ extern template hipError_t launch_col<float, 256, 4, 8, 4, 4, 2, 3, 0>(const voi
 extern template hipError_t launch_col<float, 256, 4, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 2>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 8, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 8, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 8, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
@@ -134,6 +145,7 @@ extern template hipError_t launch_col<float, 512, 2, 16, 4, 4, 1, 1, 0>(const vo
 extern template hipError_t launch_col<float, 512, 4, 8, 2, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 4, 8, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 4, 8, 4, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 4, 8, 4, 4, 2, 3, 2>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 4, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 1, 4, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 1, 4, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
@@ -145,10 +157,12 @@ extern template hipError_t launch_col<float, 1024, 1, 16, 4, 2, 1, 1, 0>(const v
 extern template hipError_t launch_col<float, 1024, 2, 4, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 2, 3, 3>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 2, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 1, 1, 1>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 1, 1, 2>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 4, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
@@ -172,14 +186,17 @@ extern template hipError_t launch_col<float, 2048, 2, 8, 4, 2, 2, 3, 0>(const vo
 extern template hipError_t launch_col<float, 2048, 2, 8, 4, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 2>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 2>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 1, 8, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 1, 16, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 1, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 2, 16, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 2, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 2>(const void*, int, int, int, const void*, hipStream_t);
 namespace {
 using T = float;
 const RowCfg kRow[] = {
@@ -194,6 +211,7 @@ const RowCfg kRow[] = {
     {128, 2, 16, 2, 16, launch_row<T, 128, 2, 16, 2, 1, 0>, 1, 0},
     {128, 4, 4, 2, 128, launch_row<T, 128, 4, 4, 2, 3, 0>, 3, 0},
     {128, 4, 4, 2, 128, launch_row<T, 128, 4, 4, 2, 3, 1>, 3, 1},
+    {128, 4, 4, 2, 128, launch_row<T, 128, 4, 4, 2, 3, 5>, 3, 5},
     {128, 4, 8, 2, 64, launch_row<T, 128, 4, 8, 2, 1, 0>, 1, 0},
     {128, 4, 8, 2, 64, launch_row<T, 128, 4, 8, 2, 1, 1>, 1, 1},
     {128, 4, 8, 2, 64, launch_row<T, 128, 4, 8, 2, 3, 0>, 3, 0},
@@ -212,12 +230,14 @@ const RowCfg kRow[] = {
     {256, 2, 16, 2, 32, launch_row<T, 256, 2, 16, 2, 1, 0>, 1, 0},
     {256, 4, 4, 2, 256, launch_row<T, 256, 4, 4, 2, 3, 0>, 3, 0},
     {256, 4, 4, 2, 256, launch_row<T, 256, 4, 4, 2, 3, 1>, 3, 1},
+    {256, 4, 4, 2, 256, launch_row<T, 256, 4, 4, 2, 3, 4>, 3, 4},
     {256, 4, 8, 2, 128, launch_row<T, 256, 4, 8, 2, 1, 0>, 1, 0},
     {256, 4, 8, 2, 128, launch_row<T, 256, 4, 8, 2, 1, 1>, 1, 1},
     {256, 4, 8, 2, 128, launch_row<T, 256, 4, 8, 2, 3, 0>, 3, 0},
     {256, 4, 8, 2, 128, launch_row<T, 256, 4, 8, 2, 3, 1>, 3, 1},
     {256, 4, 8, 4, 128, launch_row<T, 256, 4, 8, 4, 1, 0>, 1, 0},
     {256, 4, 8, 4, 128, launch_row<T, 256, 4, 8, 4, 3, 0>, 3, 0},
+    {256, 4, 8, 4, 128, launch_row<T, 256, 4, 8, 4, 3, 4>, 3, 4},
     {256, 4, 16, 2, 64, launch_row<T, 256, 4, 16, 2, 1, 0>, 1, 0},
     {256, 4, 16, 2, 64, launch_row<T, 256, 4, 16, 2, 1, 1>, 1, 1},
     {256, 4, 16, 2, 64, launch_row<T, 256, 4, 16, 2, 3, 0>, 3, 0},
@@ -230,8 +250,10 @@ const RowCfg kRow[] = {
     {512, 2, 16, 2, 64, launch_row<T, 512, 2, 16, 2, 1, 0>, 1, 0},
     {512, 4, 4, 2, 512, launch_row<T, 512, 4, 4, 2, 3, 0>, 3, 0},
     {512, 4, 4, 2, 512, launch_row<T, 512, 4, 4, 2, 3, 1>, 3, 1},
+    {512, 4, 4, 2, 512, launch_row<T, 512, 4, 4, 2, 3, 5>, 3, 5},
     {512, 4, 8, 2, 256, launch_row<T, 512, 4, 8, 2, 1, 0>, 1, 0},
     {512, 4, 8, 2, 256, launch_row<T, 512, 4, 8, 2, 1, 1>, 1, 1},
+    {512, 4, 8, 2, 256, launch_row<T, 512, 4, 8, 2, 1, 4>, 1, 4},
     {512, 4, 8, 2, 256, launch_row<T, 512, 4, 8, 2, 3, 0>, 3, 0},
     {512, 4, 8, 2, 256, launch_row<T, 512, 4, 8, 2, 3, 1>, 3, 1},
     {512, 4, 8, 4, 256, launch_row<T, 512, 4, 8, 4, 1, 0>, 1, 0},
@@ -248,15 +270,18 @@ const RowCfg kRow[] = {
     {1024, 4, 8, 2, 512, launch_row<T, 1024, 4, 8, 2, 1, 1>, 1, 1},
     {1024, 4, 8, 2, 512, launch_row<T, 1024, 4, 8, 2, 3, 0>, 3, 0},
     {1024, 4, 8, 4, 512, launch_row<T, 1024, 4, 8, 4, 1, 0>, 1, 0},
+    {1024, 4, 8, 4, 512, launch_row<T, 1024, 4, 8, 4, 1, 4>, 1, 4},
     {1024, 4, 8, 4, 512, launch_row<T, 1024, 4, 8, 4, 3, 0>, 3, 0},
     {1024, 4, 16, 2, 256, launch_row<T, 1024, 4, 16, 2, 1, 0>, 1, 0},
     {1024, 4, 16, 2, 256, launch_row<T, 1024, 4, 16, 2, 1, 1>, 1, 1},
+    {1024, 4, 16, 2, 256, launch_row<T, 1024, 4, 16, 2, 1, 4>, 1, 4},
     {1024, 4, 16, 2, 256, launch_row<T, 1024, 4, 16, 2, 3, 0>, 3, 0},
     {1024, 4, 16, 4, 256, launch_row<T, 1024, 4, 16, 4, 1, 0>, 1, 0},
     {2048, 2, 8, 2, 512, launch_row<T, 2048, 2, 8, 2, 1, 0>, 1, 0},
     {2048, 2, 16, 2, 256, launch_row<T, 2048, 2, 16, 2, 1, 0>, 1, 0},
     {2048, 4, 16, 2, 512, launch_row<T, 2048, 4, 16, 2, 1, 0>, 1, 0},
     {2048, 4, 16, 4, 512, launch_row<T, 2048, 4, 16, 4, 1, 0>, 1, 0},
+    {2048, 4, 16, 4, 512, launch_row<T, 2048, 4, 16, 4, 1, 4>, 1, 4},
 };
 const ColCfg kCol[] = {
     {32, 2, 8, 2, 2, 8, launch_col<T, 32, 2, 8, 2, 2, 2, 3, 0>, 2, 3, 0},
@@ -275,6 +300,7 @@ const ColCfg kCol[] = {
     {256, 2, 4, 2, 2, 128, launch_col<T, 256, 2, 4, 2, 2, 1, 1, 0>, 1, 1, 0},
     {256, 2, 4, 4, 2, 128, launch_col<T, 256, 2, 4, 4, 2, 2, 3, 0>, 2, 3, 0},
     {256, 2, 4, 4, 2, 128, launch_col<T, 256, 2, 4, 4, 2, 2, 3, 1>, 2, 3, 1},
+    {256, 2, 4, 4, 2, 128, launch_col<T, 256, 2, 4, 4, 2, 2, 3, 3>, 2, 3, 3},
     {256, 2, 8, 2, 2, 64, launch_col<T, 256, 2, 8, 2, 2, 1, 1, 0>, 1, 1, 0},
     {256, 2, 8, 2, 2, 64, launch_col<T, 256, 2, 8, 2, 2, 2, 3, 0>, 2, 3, 0},
     {256, 2, 8, 4, 2, 64, launch_col<T, 256, 2, 8, 4, 2, 1, 1, 0>, 1, 1, 0},
@@ -291,6 +317,7 @@ const ColCfg kCol[] = {
     {256, 4, 16, 4, 4, 64, launch_col<T, 256, 4, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 0>, 2, 3, 0},
     {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 1>, 2, 3, 1},
+    {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 2>, 2, 3, 2},
     {512, 1, 8, 2, 2, 64, launch_col<T, 512, 1, 8, 2, 2, 1, 1, 0>, 1, 1, 0},
     {512, 1, 8, 4, 2, 64, launch_col<T, 512, 1, 8, 4, 2, 1, 1, 0>, 1, 1, 0},
     {512, 1, 8, 4, 2, 64, launch_col<T, 512, 1, 8, 4, 2, 2, 3, 0>, 2, 3, 0},
@@ -312,6 +339,7 @@ const ColCfg kCol[] = {
     {512, 4, 8, 2, 4, 256, launch_col<T, 512, 4, 8, 2, 4, 2, 3, 0>, 2, 3, 0},
     {512, 4, 8, 4, 4, 256, launch_col<T, 512, 4, 8, 4, 4, 1, 1, 0>, 1, 1, 0},
     {512, 4, 8, 4, 4, 256, launch_col<T, 512, 4, 8, 4, 4, 2, 3, 0>, 2, 3, 0},
+    {512, 4, 8, 4, 4, 256, launch_col<T, 512, 4, 8, 4, 4, 2, 3, 2>, 2, 3, 2},
     {512, 4, 16, 4, 4, 128, launch_col<T, 512, 4, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {1024, 1, 4, 4, 2, 256, launch_col<T, 1024, 1, 4, 4, 2, 2, 3, 0>, 2, 3, 0},
     {1024, 1, 4, 4, 2, 256, launch_col<T, 1024, 1, 4, 4, 2, 2, 3, 1>, 2, 3, 1},
@@ -323,10 +351,12 @@ const ColCfg kCol[] = {
     {1024, 2, 4, 2, 2, 512, launch_col<T, 1024, 2, 4, 2, 2, 1, 1, 0>, 1, 1, 0},
     {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 2, 3, 0>, 2, 3, 0},
     {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 2, 3, 1>, 2, 3, 1},
+    {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 2, 3, 3>, 2, 3, 3},
     {1024, 2, 8, 2, 2, 256, launch_col<T, 1024, 2, 8, 2, 2, 1, 1, 0>, 1, 1, 0},
     {1024, 2, 8, 2, 2, 256, launch_col<T, 1024, 2, 8, 2, 2, 2, 3, 0>, 2, 3, 0},
     {1024, 2, 8, 4, 2, 256, launch_col<T, 1024, 2, 8, 4, 2, 1, 1, 0>, 1, 1, 0},
     {1024, 2, 8, 4, 2, 256, launch_col<T, 1024, 2, 8, 4, 2, 1, 1, 1>, 1, 1, 1},
+    {1024, 2, 8, 4, 2, 256, launch_col<T, 1024, 2, 8, 4, 2, 1, 1, 2>, 1, 1, 2},
     {1024, 2, 8, 4, 2, 256, launch_col<T, 1024, 2, 8, 4, 2, 2, 3, 0>, 2, 3, 0},
     {1024, 2, 8, 4, 2, 256, launch_col<T, 1024, 2, 8, 4, 2, 2, 3, 1>, 2, 3, 1},
     {1024, 2, 8, 4, 4, 256, launch_col<T, 1024, 2, 8, 4, 4, 2, 3, 0>, 2, 3, 0},
@@ -350,14 +380,17 @@ const ColCfg kCol[] = {
     {2048, 2, 8, 4, 4, 512, launch_col<T, 2048, 2, 8, 4, 4, 2, 3, 0>, 2, 3, 0},
     {2048, 2, 16, 2, 2, 256, launch_col<T, 2048, 2, 16, 2, 2, 1, 1, 0>, 1, 1, 0},
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 2>, 1, 1, 2},
     {2048, 2, 16, 4, 4, 256, launch_col<T, 2048, 2, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
+    {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 2>, 1, 1, 2},
     {4096, 1, 8, 2, 2, 512, launch_col<T, 4096, 1, 8, 2, 2, 1, 1, 0>, 1, 1, 0},
     {4096, 1, 16, 2, 2, 256, launch_col<T, 4096, 1, 16, 2, 2, 1, 1, 0>, 1, 1, 0},
     {4096, 1, 16, 4, 2, 256, launch_col<T, 4096, 1, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
     {4096, 2, 16, 2, 2, 512, launch_col<T, 4096, 2, 16, 2, 2, 1, 1, 0>, 1, 1, 0},
     {4096, 2, 16, 4, 2, 512, launch_col<T, 4096, 2, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
     {4096, 2, 16, 4, 4, 512, launch_col<T, 4096, 2, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
+    {4096, 2, 16, 4, 4, 512, launch_col<T, 4096, 2, 16, 4, 4, 1, 1, 2>, 1, 1, 2},
 };
 const Schedule kSched[] = {
     {16, 32, 4, 2, 4, 3, 8, 2, 3, 2, 0, 1},
@@ -425,6 +458,9 @@ const Schedule kSched[] = {
     {128, 256, 4, 2, 8, 3, 8, 2, 3, 2, 1, 0},
     {128, 256, 4, 2, 16, 1, 8, 1, 1, 2, 1, 0},
     {128, 256, 4, 2, 8, 1, 8, 1, 1, 2, 1, 0},
+    {128, 256, 4, 2, 4, 3, 4, 2, 3, 2, 3, 0},
+    {128, 256, 4, 2, 4, 3, 4, 2, 3, 2, 5, 0},
+    {128, 256, 4, 2, 4, 3, 4, 2, 3, 2, 7, 0},
     {128, 512, 4, 2, 8, 3, 8, 2, 3, 2, 0, 1},
     {128, 1024, 4, 2, 8, 3, 8, 2, 3, 2, 0, 1},
     {128, 2048, 4, 2, 8, 3, 8, 2, 3, 2, 0, 1},
@@ -471,6 +507,12 @@ const Schedule kSched[] = {
     {256, 512, 4, 2, 8, 3, 8, 2, 3, 2, 1, 0},
     {256, 512, 4, 2, 16, 1, 8, 1, 1, 2, 1, 0},
     {256, 512, 4, 2, 8, 1, 8, 1, 1, 2, 1, 0},
+    {256, 512, 4, 2, 4, 3, 4, 2, 3, 1, 2, 0},
+    {256, 512, 4, 2, 4, 3, 4, 2, 3, 1, 4, 0},
+    {256, 512, 4, 2, 4, 3, 4, 2, 3, 1, 6, 0},
+    {256, 512, 4, 4, 8, 3, 8, 2, 3, 4, 2, 0},
+    {256, 512, 4, 4, 8, 3, 8, 2, 3, 4, 4, 0},
+    {256, 512, 4, 4, 8, 3, 8, 2, 3, 4, 6, 0},
     {256, 1024, 4, 2, 8, 3, 8, 2, 3, 2, 0, 1},
     {256, 2048, 4, 2, 8, 3, 8, 2, 3, 2, 0, 1},
     {256, 4096, 4, 2, 8, 3, 16, 1, 1, 2, 0, 1},
@@ -517,6 +559,12 @@ const Schedule kSched[] = {
     {512, 1024, 4, 2, 8, 3, 8, 2, 3, 2, 1, 0},
     {512, 1024, 4, 2, 16, 1, 8, 1, 1, 2, 1, 0},
     {512, 1024, 4, 2, 8, 1, 8, 1, 1, 2, 1, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 2, 3, 2, 3, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 2, 3, 2, 5, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 2, 3, 2, 7, 0},
+    {512, 1024, 4, 2, 8, 1, 8, 1, 1, 2, 2, 0},
+    {512, 1024, 4, 2, 8, 1, 8, 1, 1, 2, 4, 0},
+    {512, 1024, 4, 2, 8, 1, 8, 1, 1, 2, 6, 0},
     {512, 2048, 4, 2, 8, 3, 8, 2, 3, 2, 0, 1},
     {512, 4096, 4, 2, 8, 3, 16, 1, 1, 2, 0, 1},
     {1024, 32, 4, 2, 8, 3, 8, 2, 3, 2, 0, 1},
@@ -548,6 +596,12 @@ const Schedule kSched[] = {
     {1024, 2048, 4, 4, 16, 1, 16, 1, 1, 4, 0, 0},
     {1024, 2048, 4, 2, 16, 1, 8, 1, 1, 2, 1, 0},
     {1024, 2048, 4, 2, 8, 1, 8, 1, 1, 2, 1, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 2, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 4, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 6, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 2, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 4, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 6, 0},
     {1024, 4096, 4, 2, 8, 3, 16, 1, 1, 2, 0, 1},
     {2048, 32, 2, 2, 8, 1, 8, 2, 3, 2, 0, 1},
     {2048, 64, 2, 2, 8, 1, 8, 2, 3, 2, 0, 1},
@@ -563,6 +617,9 @@ const Schedule kSched[] = {
     {2048, 4096, 4, 2, 16, 1, 16, 1, 1, 1, 0, 0},
     {2048, 4096, 2, 2, 16, 1, 16, 1, 1, 2, 0, 0},
     {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 0, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 2, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 4, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 6, 0},
 };
 }  // namespace
 

@@ -60,8 +60,8 @@ void load_table(const std::string& path, std::map<Key, const Schedule*>& into) {
   char line[512];
   while (std::fgets(line, sizeof line, f)) {
     if (line[0] == '#') continue;
-    // prec nx ny by bx row_r row_g col_r col_gf col_gi [col_w [pp]]
-    // (col_w defaults to bx, pp to 0)
+    // prec nx ny by bx row_r row_g col_r col_gf col_gi [col_w [options]]
+    // (col_w defaults to bx, options to 0)
     int prec, nx, ny, v[9];
     const int got = std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d", &prec, &nx, &ny, &v[0], &v[1], &v[2],
                                 &v[3], &v[4], &v[5], &v[6], &v[7], &v[8]);
@@ -131,7 +131,7 @@ const RowCfg* find_row_cfg(int precision, const Schedule* s) {
   const RowCfg* c = precision == 64 ? row_cfgs_f64(&n) : row_cfgs_f32(&n);
   for (int i = 0; i < n; ++i)
     if (c[i].nx2 == s->nx2 && c[i].rb == s->by && c[i].r == s->row_r && c[i].bx == s->bx && c[i].g == s->row_g &&
-        c[i].pp == s->pp)
+        c[i].pp == (s->pp & kRowOptions))
       return &c[i];
   return nullptr;
 }
@@ -141,7 +141,7 @@ const ColCfg* find_col_cfg(int precision, const Schedule* s) {
   const ColCfg* c = precision == 64 ? col_cfgs_f64(&n) : col_cfgs_f32(&n);
   for (int i = 0; i < n; ++i)
     if (c[i].ny == s->ny && c[i].w == s->col_w && c[i].bx == s->bx && c[i].r == s->col_r && c[i].by == s->by &&
-        c[i].gf == s->col_gf && c[i].gi == s->col_gi && c[i].pp == s->pp)
+        c[i].gf == s->col_gf && c[i].gi == s->col_gi && c[i].pp == (s->pp & kColOptions))
       return &c[i];
   return nullptr;
 }

@@ -14,18 +14,25 @@ using ColLaunch = hipError_t (*)(const void* descs, int ninst, int tiles, int st
 using ConvLaunch = hipError_t (*)(const void* src, void* dst, int nx, int ny, int by, int bx, hipStream_t s);
 using IfacLaunch = hipError_t (*)(const void* desc, int nx2, int ny, int w, hipStream_t s);
 
+// Schedule option bits (the kernels' PP template argument): 1 double-buffered
+// LDS exchanges (both passes), 2 streaming column-pass stores, 4 streaming
+// row-pass stores (otherwise device-scope write-through).  A row kernel is
+// instantiated for the bits it reads only, as is a column kernel.
+constexpr int kOptLdsDouble = 1, kOptColStream = 2, kOptRowStream = 4;
+constexpr int kRowOptions = kOptLdsDouble | kOptRowStream, kColOptions = kOptLdsDouble | kOptColStream;
+
 struct RowCfg {
   int nx2, rb, r, bx, threads;
   RowLaunch launch;
-  int g, pp;   // pp: double-buffered LDS exchanges
+  int g, pp;   // pp: option bits of the row kernel
 };
 struct ColCfg {
   int ny, w, r, by, bx, threads;   // w: columns per block (divides the brick width bx)
   ColLaunch launch;
-  int gf, gi, pp;
+  int gf, gi, pp;   // pp: option bits of the column kernel
 };
 struct Schedule {
-  int nx2, ny, by, bx, row_r, row_g, col_r, col_gf, col_gi, col_w, pp, is_default;
+  int nx2, ny, by, bx, row_r, row_g, col_r, col_gf, col_gi, col_w, pp, is_default;   // pp: option bits
 };
 
 // Schedules instantiated for (precision, nx2, ny); index 0 is the default.

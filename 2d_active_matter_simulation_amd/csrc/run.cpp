@@ -467,7 +467,7 @@ static void fill_schedule(const Schedule* s, af_schedule* out) {
   out->col_fwd_group = s->col_gf;
   out->col_inv_group = s->col_gi;
   out->col_width = s->col_w;
-  out->lds_double = s->pp;
+  out->options = s->pp;
 }
 
 int af_schedule_count(int precision, int nx, int ny) { return schedule_count(precision, nx / 2, ny); }

@@ -47,6 +47,13 @@ template <class T> using gptr = T*;
 #endif
 template <class T> __device__ __forceinline__ gptr<T> glob(T* p) { return (gptr<T>)p; }
 
+// Cache-policy bits of gfx950 buffer stores (the instruction's sc0 / nt / sc1
+// fields): a device-scope store (sc1) is written through the XCD's L2, so the
+// end-of-kernel write-back has nothing left to flush; a streaming store (nt)
+// is marked for early eviction.  Which one pays depends on the pass and the
+// workload size, so it is a schedule option (measured by the autotuner).
+constexpr int kStoreDefault = 0, kStoreNt = 2, kStoreSc1 = 16;
+
 // Buffer-resource view of a complex field: the base lives in a 4-SGPR buffer
 // descriptor, a lane's byte offset in a VGPR and a uniform byte offset (e.g. a
 // row group's) in an SGPR, so an access costs no per-lane address arithmetic.
@@ -74,16 +81,19 @@ struct BufField {
     else
       return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(rs, vbyte, sbyte, 0));
   }
+  // CP: cache-policy bits of 8-byte stores (16-byte stores take the default)
+  template <int CP = kStoreDefault>
   __device__ __forceinline__ void st(unsigned vbyte, unsigned sbyte, T v) const {
     if constexpr (sizeof(T) == 8)
       __builtin_amdgcn_raw_buffer_store_b64(
-          __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(rs, 0, 0, 0)), v), rs, vbyte, sbyte, 0);
+          __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(rs, 0, 0, 0)), v), rs, vbyte, sbyte, CP);
     else   // 16-byte stores go through a global store (see above)
       *(gptr<T>)(base + sbyte + vbyte) = v;
   }
 #else   // host pass of hipcc: never executed on the host
   __device__ explicit BufField(const T*) {}
   __device__ T ld(unsigned, unsigned) const { return T{}; }
+  template <int CP = kStoreDefault>
   __device__ void st(unsigned, unsigned, T) const {}
 #endif
 };
@@ -189,8 +199,10 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red /* LDS, N
 // ---------------------------------------------------------------------------
 // Row pass.  Block = RB rows (one brick row) x TR = NX2/R lanes per row.
 // G = 3: the three C2R transforms share their LDS exchanges (and the two
-// R2C transforms theirs); G = 1: one field at a time (less LDS).  PP: two
-// sets of LDS areas, one barrier per transform exchange (more LDS).
+// R2C transforms theirs); G = 1: one field at a time (less LDS).
+// PP, option bits: 1 = two sets of LDS areas, one barrier per transform
+// exchange (more LDS); 4 = streaming stores of P, Q (otherwise device-scope
+// write-through stores).
 // ---------------------------------------------------------------------------
 template <class T, int NX2, int RB, int R, int BX, int G, int PP>
 __global__ void __launch_bounds__(RB * (NX2 / R))
@@ -200,7 +212,8 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   static_assert(NX2 % BX == 0 && TR % BX == 0, "row lanes must cover whole bricks");
   static_assert(G == 1 || G == 3, "group of 1 or 3 fields");
   constexpr int GR = G == 3 ? 2 : 1;   // R2C group
-  using View = LdsView<T, NX2, RB, R, G, PP>;
+  using View = LdsView<T, NX2, RB, R, G, PP & 1>;
+  constexpr int kCP = (PP & 4) ? kStoreNt : kStoreSc1;
   __shared__ cx<T> lds_mem[View::kElems];
   __shared__ cx<T> tw_h[TwLayout<NX2, R>::kAlloc];   // the half-length transforms
   __shared__ cx<T> tw_n[NX2];          // exp(-2 pi i k / Nx), k < Nx/2, the even/odd split
@@ -350,8 +363,8 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   for (int r = 0; r < R; ++r) {
     const int k = j + r * TR;
     const T kx = tkx * (T)k;
-    gpm.st(vb, rb(r), cx<T>{kx * f2[0][r].im, -kx * f2[0][r].re});   // -i kx X1
-    gqm.st(vb, rb(r), f2[1][r]);
+    gpm.template st<kCP>(vb, rb(r), cx<T>{kx * f2[0][r].im, -kx * f2[0][r].re});   // -i kx X1
+    gqm.template st<kCP>(vb, rb(r), f2[1][r]);
   }
 
   double sums[3] = {esum, gxs, gys};
@@ -371,6 +384,7 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 // stage 1..4: full stage.  The stage is a template argument: each stage's
 // kernel keeps only the state it reads live (fewer registers, more waves).  GF = 2 / GI = 3: the forward (P, Q) and inverse
 // (v_x, v_y, omega) transforms share LDS exchanges; 1: one at a time.
+// PP, option bits: 1 = double-buffered LDS exchanges; 2 = streaming stores.
 // ---------------------------------------------------------------------------
 template <class T, int NY, int W, int R, int BY, int BX, int GF, int GI, int PP, int stage>
 __global__ void __launch_bounds__(W * (NY / R))
@@ -379,7 +393,8 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   constexpr int NT = W * TC;
   constexpr int GMAX = GF > GI ? GF : GI;
   static_assert((GF == 1 || GF == 2) && (GI == 1 || GI == 3), "field groups");
-  using View = LdsView<T, NY, W, R, GMAX, PP>;
+  using View = LdsView<T, NY, W, R, GMAX, PP & 1>;
+  constexpr int kCP = (PP & 2) ? kStoreNt : kStoreDefault;
   __shared__ cx<T> lds_mem[View::kElems];
   __shared__ cx<T> tw_y[TwLayout<NY, R>::kAlloc];
   __shared__ double mean_in[7];   // tile 0: sum|u|^2, sum Cx, sum Cy, M, M_old
@@ -410,7 +425,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   const unsigned vb = addr(j) * ESZ;
   const unsigned rstride = (unsigned)(TC / BY) * brow * ESZ;   // bytes, uniform
   auto ld = [&](const cx<T>* f, int r) -> cx<T> { return BufField<cx<T>>(f).ld(vb, (unsigned)r * rstride); };
-  auto st = [&](cx<T>* f, int r, cx<T> v) { BufField<cx<T>>(f).st(vb, (unsigned)r * rstride, v); };
+  auto st = [&](cx<T>* f, int r, cx<T> v) { BufField<cx<T>>(f).template st<kCP>(vb, (unsigned)r * rstride, v); };
 
   const double dt = d.dt, dt2 = d.dt2, dt3 = d.dt3, dt6 = d.dt6;
   const bool col0 = (x == 0);

@@ -117,8 +117,12 @@ typedef struct {
   int row_radix, row_group; /* row pass radix, fields per LDS exchange        */
   int col_radix, col_fwd_group, col_inv_group;
   int col_width;            /* columns per column-pass block (divides bx)     */
-  int lds_double;           /* 1: two sets of LDS areas, one barrier per FFT  *
-                             * exchange instead of two (more LDS per block)   */
+  int options;              /* bits: 1 two sets of LDS areas, one barrier per *
+                             * FFT exchange instead of two (more LDS per      *
+                             * block); 2 streaming (nt) column-pass stores;   *
+                             * 4 streaming row-pass stores instead of device- *
+                             * scope write-through (sc1).  FP32 only for 2/4: *
+                             * FP64 stores take the default policy.           */
 } af_schedule;
 int af_schedule_count(int precision, int nx, int ny);
 af_status af_schedule_get(int precision, int nx, int ny, int index, af_schedule* out);

@@ -86,8 +86,11 @@ def test_tuned_tables_name_instantiated_schedules(af):
 
 
 def test_schedule_struct_fields(af):
-    """af_schedule carries the column-block width (divides the brick width)."""
+    """af_schedule carries the column-block width (divides the brick width)
+    and the option bits (LDS double buffering; streaming stores, FP32 only)."""
     for prec in (32, 64):
         for sc in af.schedules(prec, 1024):
-            by, bx, rr, rg, cr, gf, gi, cw, pp = sc
-            assert bx % cw == 0 and rr in (4, 8, 16) and cr in (4, 8, 16) and pp in (0, 1)
+            by, bx, rr, rg, cr, gf, gi, cw, opts = sc
+            assert bx % cw == 0 and rr in (4, 8, 16) and cr in (4, 8, 16)
+            assert 0 <= opts < 8 and (prec == 32 or opts in (0, 1))
+    assert any(sc[8] & 6 for sc in af.schedules(32, 1024))
