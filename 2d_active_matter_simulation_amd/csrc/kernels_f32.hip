@@ -4,7 +4,8 @@
 // Row kernel:    (Nx/2, RB = brick height, R, BX = brick width, G, PP)
 // Column kernel: (Ny, W = columns per block, R, BY, BX = brick height/width, GF, GI, PP)
 // Schedule:      (Nx/2, Ny, BY, BX, row R, row G, col R, col GF, col GI, col W, PP, default)
-// PP: option bits (1 double-buffered LDS, 2 / 4 streaming column / row stores)
+// PP: option bits (1 double-buffered LDS, 2 / 4 streaming column / row stores,
+// 8 streaming column-pass P, Q loads)
 #include "kernels_inst.hpp"
 
 namespace af {
@@ -107,6 +108,8 @@ extern template hipError_t launch_col<float, 256, 2, 4, 2, 2, 1, 1, 0>(const voi
 extern template hipError_t launch_col<float, 256, 2, 4, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 2, 4, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 2, 4, 4, 2, 2, 3, 3>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 256, 2, 4, 4, 2, 2, 3, 9>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 256, 2, 4, 4, 2, 2, 3, 11>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 2, 8, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 2, 8, 2, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 2, 8, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
@@ -124,6 +127,8 @@ extern template hipError_t launch_col<float, 256, 4, 16, 4, 4, 1, 1, 0>(const vo
 extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 2>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 8>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 10>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 8, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 8, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 8, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
@@ -146,6 +151,8 @@ extern template hipError_t launch_col<float, 512, 4, 8, 2, 4, 2, 3, 0>(const voi
 extern template hipError_t launch_col<float, 512, 4, 8, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 4, 8, 4, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 4, 8, 4, 4, 2, 3, 2>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 4, 8, 4, 4, 2, 3, 8>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 4, 8, 4, 4, 2, 3, 10>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 4, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 1, 4, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 1, 4, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
@@ -158,11 +165,15 @@ extern template hipError_t launch_col<float, 1024, 2, 4, 2, 2, 1, 1, 0>(const vo
 extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 2, 3, 3>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 2, 3, 9>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 2, 3, 11>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 2, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 1, 1, 1>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 1, 1, 2>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 1, 1, 8>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 1, 1, 10>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 8, 4, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
@@ -187,9 +198,13 @@ extern template hipError_t launch_col<float, 2048, 2, 8, 4, 4, 2, 3, 0>(const vo
 extern template hipError_t launch_col<float, 2048, 2, 16, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 2>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 8>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 10>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 2>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 8>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 10>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 1, 8, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 1, 16, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 1, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
@@ -197,6 +212,8 @@ extern template hipError_t launch_col<float, 4096, 2, 16, 2, 2, 1, 1, 0>(const v
 extern template hipError_t launch_col<float, 4096, 2, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 2>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 8>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 10>(const void*, int, int, int, const void*, hipStream_t);
 namespace {
 using T = float;
 const RowCfg kRow[] = {
@@ -301,6 +318,8 @@ const ColCfg kCol[] = {
     {256, 2, 4, 4, 2, 128, launch_col<T, 256, 2, 4, 4, 2, 2, 3, 0>, 2, 3, 0},
     {256, 2, 4, 4, 2, 128, launch_col<T, 256, 2, 4, 4, 2, 2, 3, 1>, 2, 3, 1},
     {256, 2, 4, 4, 2, 128, launch_col<T, 256, 2, 4, 4, 2, 2, 3, 3>, 2, 3, 3},
+    {256, 2, 4, 4, 2, 128, launch_col<T, 256, 2, 4, 4, 2, 2, 3, 9>, 2, 3, 9},
+    {256, 2, 4, 4, 2, 128, launch_col<T, 256, 2, 4, 4, 2, 2, 3, 11>, 2, 3, 11},
     {256, 2, 8, 2, 2, 64, launch_col<T, 256, 2, 8, 2, 2, 1, 1, 0>, 1, 1, 0},
     {256, 2, 8, 2, 2, 64, launch_col<T, 256, 2, 8, 2, 2, 2, 3, 0>, 2, 3, 0},
     {256, 2, 8, 4, 2, 64, launch_col<T, 256, 2, 8, 4, 2, 1, 1, 0>, 1, 1, 0},
@@ -318,6 +337,8 @@ const ColCfg kCol[] = {
     {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 0>, 2, 3, 0},
     {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 1>, 2, 3, 1},
     {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 2>, 2, 3, 2},
+    {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 8>, 2, 3, 8},
+    {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 10>, 2, 3, 10},
     {512, 1, 8, 2, 2, 64, launch_col<T, 512, 1, 8, 2, 2, 1, 1, 0>, 1, 1, 0},
     {512, 1, 8, 4, 2, 64, launch_col<T, 512, 1, 8, 4, 2, 1, 1, 0>, 1, 1, 0},
     {512, 1, 8, 4, 2, 64, launch_col<T, 512, 1, 8, 4, 2, 2, 3, 0>, 2, 3, 0},
@@ -340,6 +361,8 @@ const ColCfg kCol[] = {
     {512, 4, 8, 4, 4, 256, launch_col<T, 512, 4, 8, 4, 4, 1, 1, 0>, 1, 1, 0},
     {512, 4, 8, 4, 4, 256, launch_col<T, 512, 4, 8, 4, 4, 2, 3, 0>, 2, 3, 0},
     {512, 4, 8, 4, 4, 256, launch_col<T, 512, 4, 8, 4, 4, 2, 3, 2>, 2, 3, 2},
+    {512, 4, 8, 4, 4, 256, launch_col<T, 512, 4, 8, 4, 4, 2, 3, 8>, 2, 3, 8},
+    {512, 4, 8, 4, 4, 256, launch_col<T, 512, 4, 8, 4, 4, 2, 3, 10>, 2, 3, 10},
     {512, 4, 16, 4, 4, 128, launch_col<T, 512, 4, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {1024, 1, 4, 4, 2, 256, launch_col<T, 1024, 1, 4, 4, 2, 2, 3, 0>, 2, 3, 0},
     {1024, 1, 4, 4, 2, 256, launch_col<T, 1024, 1, 4, 4, 2, 2, 3, 1>, 2, 3, 1},
@@ -352,11 +375,15 @@ const ColCfg kCol[] = {
     {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 2, 3, 0>, 2, 3, 0},
     {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 2, 3, 1>, 2, 3, 1},
     {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 2, 3, 3>, 2, 3, 3},
+    {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 2, 3, 9>, 2, 3, 9},
+    {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 2, 3, 11>, 2, 3, 11},
     {1024, 2, 8, 2, 2, 256, launch_col<T, 1024, 2, 8, 2, 2, 1, 1, 0>, 1, 1, 0},
     {1024, 2, 8, 2, 2, 256, launch_col<T, 1024, 2, 8, 2, 2, 2, 3, 0>, 2, 3, 0},
     {1024, 2, 8, 4, 2, 256, launch_col<T, 1024, 2, 8, 4, 2, 1, 1, 0>, 1, 1, 0},
     {1024, 2, 8, 4, 2, 256, launch_col<T, 1024, 2, 8, 4, 2, 1, 1, 1>, 1, 1, 1},
     {1024, 2, 8, 4, 2, 256, launch_col<T, 1024, 2, 8, 4, 2, 1, 1, 2>, 1, 1, 2},
+    {1024, 2, 8, 4, 2, 256, launch_col<T, 1024, 2, 8, 4, 2, 1, 1, 8>, 1, 1, 8},
+    {1024, 2, 8, 4, 2, 256, launch_col<T, 1024, 2, 8, 4, 2, 1, 1, 10>, 1, 1, 10},
     {1024, 2, 8, 4, 2, 256, launch_col<T, 1024, 2, 8, 4, 2, 2, 3, 0>, 2, 3, 0},
     {1024, 2, 8, 4, 2, 256, launch_col<T, 1024, 2, 8, 4, 2, 2, 3, 1>, 2, 3, 1},
     {1024, 2, 8, 4, 4, 256, launch_col<T, 1024, 2, 8, 4, 4, 2, 3, 0>, 2, 3, 0},
@@ -381,9 +408,13 @@ const ColCfg kCol[] = {
     {2048, 2, 16, 2, 2, 256, launch_col<T, 2048, 2, 16, 2, 2, 1, 1, 0>, 1, 1, 0},
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 2>, 1, 1, 2},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 8>, 1, 1, 8},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 10>, 1, 1, 10},
     {2048, 2, 16, 4, 4, 256, launch_col<T, 2048, 2, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 2>, 1, 1, 2},
+    {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 8>, 1, 1, 8},
+    {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 10>, 1, 1, 10},
     {4096, 1, 8, 2, 2, 512, launch_col<T, 4096, 1, 8, 2, 2, 1, 1, 0>, 1, 1, 0},
     {4096, 1, 16, 2, 2, 256, launch_col<T, 4096, 1, 16, 2, 2, 1, 1, 0>, 1, 1, 0},
     {4096, 1, 16, 4, 2, 256, launch_col<T, 4096, 1, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
@@ -391,6 +422,8 @@ const ColCfg kCol[] = {
     {4096, 2, 16, 4, 2, 512, launch_col<T, 4096, 2, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
     {4096, 2, 16, 4, 4, 512, launch_col<T, 4096, 2, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {4096, 2, 16, 4, 4, 512, launch_col<T, 4096, 2, 16, 4, 4, 1, 1, 2>, 1, 1, 2},
+    {4096, 2, 16, 4, 4, 512, launch_col<T, 4096, 2, 16, 4, 4, 1, 1, 8>, 1, 1, 8},
+    {4096, 2, 16, 4, 4, 512, launch_col<T, 4096, 2, 16, 4, 4, 1, 1, 10>, 1, 1, 10},
 };
 const Schedule kSched[] = {
     {16, 32, 4, 2, 4, 3, 8, 2, 3, 2, 0, 1},
@@ -461,6 +494,10 @@ const Schedule kSched[] = {
     {128, 256, 4, 2, 4, 3, 4, 2, 3, 2, 3, 0},
     {128, 256, 4, 2, 4, 3, 4, 2, 3, 2, 5, 0},
     {128, 256, 4, 2, 4, 3, 4, 2, 3, 2, 7, 0},
+    {128, 256, 4, 2, 4, 3, 4, 2, 3, 2, 9, 0},
+    {128, 256, 4, 2, 4, 3, 4, 2, 3, 2, 11, 0},
+    {128, 256, 4, 2, 4, 3, 4, 2, 3, 2, 13, 0},
+    {128, 256, 4, 2, 4, 3, 4, 2, 3, 2, 15, 0},
     {128, 512, 4, 2, 8, 3, 8, 2, 3, 2, 0, 1},
     {128, 1024, 4, 2, 8, 3, 8, 2, 3, 2, 0, 1},
     {128, 2048, 4, 2, 8, 3, 8, 2, 3, 2, 0, 1},
@@ -510,9 +547,17 @@ const Schedule kSched[] = {
     {256, 512, 4, 2, 4, 3, 4, 2, 3, 1, 2, 0},
     {256, 512, 4, 2, 4, 3, 4, 2, 3, 1, 4, 0},
     {256, 512, 4, 2, 4, 3, 4, 2, 3, 1, 6, 0},
+    {256, 512, 4, 2, 4, 3, 4, 2, 3, 1, 8, 0},
+    {256, 512, 4, 2, 4, 3, 4, 2, 3, 1, 10, 0},
+    {256, 512, 4, 2, 4, 3, 4, 2, 3, 1, 12, 0},
+    {256, 512, 4, 2, 4, 3, 4, 2, 3, 1, 14, 0},
     {256, 512, 4, 4, 8, 3, 8, 2, 3, 4, 2, 0},
     {256, 512, 4, 4, 8, 3, 8, 2, 3, 4, 4, 0},
     {256, 512, 4, 4, 8, 3, 8, 2, 3, 4, 6, 0},
+    {256, 512, 4, 4, 8, 3, 8, 2, 3, 4, 8, 0},
+    {256, 512, 4, 4, 8, 3, 8, 2, 3, 4, 10, 0},
+    {256, 512, 4, 4, 8, 3, 8, 2, 3, 4, 12, 0},
+    {256, 512, 4, 4, 8, 3, 8, 2, 3, 4, 14, 0},
     {256, 1024, 4, 2, 8, 3, 8, 2, 3, 2, 0, 1},
     {256, 2048, 4, 2, 8, 3, 8, 2, 3, 2, 0, 1},
     {256, 4096, 4, 2, 8, 3, 16, 1, 1, 2, 0, 1},
@@ -562,9 +607,17 @@ const Schedule kSched[] = {
     {512, 1024, 4, 2, 4, 3, 4, 2, 3, 2, 3, 0},
     {512, 1024, 4, 2, 4, 3, 4, 2, 3, 2, 5, 0},
     {512, 1024, 4, 2, 4, 3, 4, 2, 3, 2, 7, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 2, 3, 2, 9, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 2, 3, 2, 11, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 2, 3, 2, 13, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 2, 3, 2, 15, 0},
     {512, 1024, 4, 2, 8, 1, 8, 1, 1, 2, 2, 0},
     {512, 1024, 4, 2, 8, 1, 8, 1, 1, 2, 4, 0},
     {512, 1024, 4, 2, 8, 1, 8, 1, 1, 2, 6, 0},
+    {512, 1024, 4, 2, 8, 1, 8, 1, 1, 2, 8, 0},
+    {512, 1024, 4, 2, 8, 1, 8, 1, 1, 2, 10, 0},
+    {512, 1024, 4, 2, 8, 1, 8, 1, 1, 2, 12, 0},
+    {512, 1024, 4, 2, 8, 1, 8, 1, 1, 2, 14, 0},
     {512, 2048, 4, 2, 8, 3, 8, 2, 3, 2, 0, 1},
     {512, 4096, 4, 2, 8, 3, 16, 1, 1, 2, 0, 1},
     {1024, 32, 4, 2, 8, 3, 8, 2, 3, 2, 0, 1},
@@ -599,9 +652,17 @@ const Schedule kSched[] = {
     {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 2, 0},
     {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 4, 0},
     {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 6, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 8, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 10, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 12, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 14, 0},
     {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 2, 0},
     {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 4, 0},
     {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 6, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 8, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 10, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 12, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 14, 0},
     {1024, 4096, 4, 2, 8, 3, 16, 1, 1, 2, 0, 1},
     {2048, 32, 2, 2, 8, 1, 8, 2, 3, 2, 0, 1},
     {2048, 64, 2, 2, 8, 1, 8, 2, 3, 2, 0, 1},
@@ -620,6 +681,10 @@ const Schedule kSched[] = {
     {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 2, 0},
     {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 4, 0},
     {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 6, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 8, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 10, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 12, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 14, 0},
 };
 }  // namespace
 

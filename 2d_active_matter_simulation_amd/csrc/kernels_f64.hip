@@ -4,7 +4,8 @@
 // Row kernel:    (Nx/2, RB = brick height, R, BX = brick width, G, PP)
 // Column kernel: (Ny, W = columns per block, R, BY, BX = brick height/width, GF, GI, PP)
 // Schedule:      (Nx/2, Ny, BY, BX, row R, row G, col R, col GF, col GI, col W, PP, default)
-// PP: option bits (1 double-buffered LDS, 2 / 4 streaming column / row stores)
+// PP: option bits (1 double-buffered LDS, 2 / 4 streaming column / row stores,
+// 8 streaming column-pass P, Q loads)
 #include "kernels_inst.hpp"
 
 namespace af {

@@ -47,12 +47,14 @@ template <class T> using gptr = T*;
 #endif
 template <class T> __device__ __forceinline__ gptr<T> glob(T* p) { return (gptr<T>)p; }
 
-// Cache-policy bits of gfx950 buffer stores (the instruction's sc0 / nt / sc1
+// Cache-policy bits of gfx950 buffer instructions (their sc0 / nt / sc1
 // fields): a device-scope store (sc1) is written through the XCD's L2, so the
-// end-of-kernel write-back has nothing left to flush; a streaming store (nt)
-// is marked for early eviction.  Which one pays depends on the pass and the
-// workload size, so it is a schedule option (measured by the autotuner).
-constexpr int kStoreDefault = 0, kStoreNt = 2, kStoreSc1 = 16;
+// end-of-kernel write-back has nothing left to flush; a streaming access (nt)
+// marks its lines for early eviction.  Measured on the stage kernels: the row
+// pass's loads (read once) and stores gain from nt loads and sc1 stores at
+// every size; for the column pass's stores and its P, Q loads the better
+// choice depends on the size and batch, so those are schedule options.
+constexpr int kCacheDefault = 0, kCacheNt = 2, kCacheSc1 = 16;
 
 // Buffer-resource view of a complex field: the base lives in a 4-SGPR buffer
 // descriptor, a lane's byte offset in a VGPR and a uniform byte offset (e.g. a
@@ -72,17 +74,18 @@ struct BufField {
   __device__ __forceinline__ explicit BufField(const T* p)
       : rs(__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(p), 0, 0x7fffffff, 0x00020000)),
         base((gptr<char>)glob(const_cast<T*>(p))) {}
+  template <int CP = kCacheDefault>
   __device__ __forceinline__ T ld(unsigned vbyte, unsigned sbyte) const {
     static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "float, complex float or double");
     if constexpr (sizeof(T) == 4)
-      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, vbyte, sbyte, 0));
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, vbyte, sbyte, CP));
     else if constexpr (sizeof(T) == 8)
-      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, vbyte, sbyte, 0));
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, vbyte, sbyte, CP));
     else
-      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(rs, vbyte, sbyte, 0));
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(rs, vbyte, sbyte, CP));
   }
   // CP: cache-policy bits of 8-byte stores (16-byte stores take the default)
-  template <int CP = kStoreDefault>
+  template <int CP = kCacheDefault>
   __device__ __forceinline__ void st(unsigned vbyte, unsigned sbyte, T v) const {
     if constexpr (sizeof(T) == 8)
       __builtin_amdgcn_raw_buffer_store_b64(
@@ -92,8 +95,9 @@ struct BufField {
   }
 #else   // host pass of hipcc: never executed on the host
   __device__ explicit BufField(const T*) {}
+  template <int CP = kCacheDefault>
   __device__ T ld(unsigned, unsigned) const { return T{}; }
-  template <int CP = kStoreDefault>
+  template <int CP = kCacheDefault>
   __device__ void st(unsigned, unsigned, T) const {}
 #endif
 };
@@ -202,7 +206,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red /* LDS, N
 // R2C transforms theirs); G = 1: one field at a time (less LDS).
 // PP, option bits: 1 = two sets of LDS areas, one barrier per transform
 // exchange (more LDS); 4 = streaming stores of P, Q (otherwise device-scope
-// write-through stores).
+// write-through stores).  U, V, W are read with streaming loads.
 // ---------------------------------------------------------------------------
 template <class T, int NX2, int RB, int R, int BX, int G, int PP>
 __global__ void __launch_bounds__(RB * (NX2 / R))
@@ -213,7 +217,7 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   static_assert(G == 1 || G == 3, "group of 1 or 3 fields");
   constexpr int GR = G == 3 ? 2 : 1;   // R2C group
   using View = LdsView<T, NX2, RB, R, G, PP & 1>;
-  constexpr int kCP = (PP & 4) ? kStoreNt : kStoreSc1;
+  constexpr int kCP = (PP & 4) ? kCacheNt : kCacheSc1;
   __shared__ cx<T> lds_mem[View::kElems];
   __shared__ cx<T> tw_h[TwLayout<NX2, R>::kAlloc];   // the half-length transforms
   __shared__ cx<T> tw_n[NX2];          // exp(-2 pi i k / Nx), k < Nx/2, the even/odd split
@@ -244,9 +248,9 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     const BufField<cx<T>> gum(d.um), gvm(d.vm), gwm(d.wm);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      f3[0][r] = gum.ld(vb, rb(r));
-      f3[1][r] = gvm.ld(vb, rb(r));
-      f3[2][r] = gwm.ld(vb, rb(r));
+      f3[0][r] = gum.template ld<kCacheNt>(vb, rb(r));
+      f3[1][r] = gvm.template ld<kCacheNt>(vb, rb(r));
+      f3[2][r] = gwm.template ld<kCacheNt>(vb, rb(r));
     }
   }
   stage_fft_twiddles<T, NT, NX2, R>(tw_h, tw_of(pool, NX2));
@@ -384,7 +388,8 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 // stage 1..4: full stage.  The stage is a template argument: each stage's
 // kernel keeps only the state it reads live (fewer registers, more waves).  GF = 2 / GI = 3: the forward (P, Q) and inverse
 // (v_x, v_y, omega) transforms share LDS exchanges; 1: one at a time.
-// PP, option bits: 1 = double-buffered LDS exchanges; 2 = streaming stores.
+// PP, option bits: 1 = double-buffered LDS exchanges; 2 = streaming stores;
+// 8 = streaming loads of P, Q.
 // ---------------------------------------------------------------------------
 template <class T, int NY, int W, int R, int BY, int BX, int GF, int GI, int PP, int stage>
 __global__ void __launch_bounds__(W * (NY / R))
@@ -394,7 +399,8 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   constexpr int GMAX = GF > GI ? GF : GI;
   static_assert((GF == 1 || GF == 2) && (GI == 1 || GI == 3), "field groups");
   using View = LdsView<T, NY, W, R, GMAX, PP & 1>;
-  constexpr int kCP = (PP & 2) ? kStoreNt : kStoreDefault;
+  constexpr int kCP = (PP & 2) ? kCacheNt : kCacheDefault;
+  constexpr int kPqCP = (PP & 8) ? kCacheNt : kCacheDefault;
   __shared__ cx<T> lds_mem[View::kElems];
   __shared__ cx<T> tw_y[TwLayout<NY, R>::kAlloc];
   __shared__ double mean_in[7];   // tile 0: sum|u|^2, sum Cx, sum Cy, M, M_old
@@ -450,8 +456,8 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     cx<T> pq[2][R], wv[R], av[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      pq[0][r] = ld(d.pm, r);
-      pq[1][r] = ld(d.qm, r);
+      pq[0][r] = BufField<cx<T>>(d.pm).template ld<kPqCP>(vb, (unsigned)r * rstride);
+      pq[1][r] = BufField<cx<T>>(d.qm).template ld<kPqCP>(vb, (unsigned)r * rstride);
     }
     // Tile 0: row-block partials for the mean ODE (summed in a fixed order)
     // and the mean state, parked in LDS until the ODE so that they occupy no

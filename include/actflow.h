@@ -121,8 +121,9 @@ typedef struct {
                              * FFT exchange instead of two (more LDS per      *
                              * block); 2 streaming (nt) column-pass stores;   *
                              * 4 streaming row-pass stores instead of device- *
-                             * scope write-through (sc1).  FP32 only for 2/4: *
-                             * FP64 stores take the default policy.           */
+                             * scope write-through (sc1); 8 streaming column- *
+                             * pass loads of P, Q.  2 and 4 act on FP32 runs  *
+                             * only (FP64 stores take the default policy).    */
 } af_schedule;
 int af_schedule_count(int precision, int nx, int ny);
 af_status af_schedule_get(int precision, int nx, int ny, int index, af_schedule* out);
