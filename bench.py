@@ -223,11 +223,24 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip FP64 / 2048^2 / 4096^2 lines")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble and batched legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the served-workload leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device-to-device copies of 256 MiB (the reference kernel for scaling "
+                         "rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(instances shard across GPUs without communication; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        n = 64 << 20   # floats: 256 MiB per buffer
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"copied 5 x {n * 4} bytes", flush=True)
+        return
     af = importlib.import_module("2d_active_matter_simulation_amd")
     af.lib()
 
