@@ -34,11 +34,13 @@ def lds_col(ny, W, R, GF, GI, esz, pp=0):
 
 # Option bits of a schedule (its last field, the kernels' PP argument):
 # 1 double-buffered LDS exchanges, 2 streaming column-pass stores, 4 streaming
-# row-pass stores, 8 streaming column-pass loads of P, Q.  The row kernel reads
-# bits 1 and 4, the column kernel 1, 2 and 8.
-ROW_OPTS, COL_OPTS = 5, 11
-# FP32 schedules that also get the cache-policy variants (options | 2 ... 14):
-# the fastest single-run and batched schedules of each size.
+# row-pass stores, 8 streaming column-pass loads of P, Q, 16 integrating-factor
+# table at radix 16 (FP32).  The row kernel reads bits 1 and 4, the column
+# kernel 1, 2, 8 and 16.
+ROW_OPTS, COL_OPTS = 5, 27
+# FP32 schedules that also get the cache-policy variants (options | 2 ... 14,
+# and | 16 ... 30 for radix-16 column passes): the fastest single-run and
+# batched schedules of each size.
 STORE_VARIANT_BASES = {
     256: [(4, 2, 4, 3, 4, 2, 3, 2, 1)],
     512: [(4, 2, 4, 3, 4, 2, 3, 1, 0), (4, 4, 8, 3, 8, 2, 3, 4, 0)],
@@ -94,7 +96,9 @@ def candidates(prec, n):
     else:
         alts += [(4, 2, 4, 3, 4, 2, 3, 2, 1), (4, 1, 8, 3, 8, 2, 3, 1, 1), (4, 2, 8, 1, 8, 1, 1, 1, 1)]
     if prec == 32:
-        alts += [b[:8] + (b[8] | o,) for b in STORE_VARIANT_BASES.get(n, []) for o in range(2, 16, 2)]
+        for b in STORE_VARIANT_BASES.get(n, []):
+            opts = list(range(2, 16, 2)) + ([16 + o for o in range(0, 16, 2)] if b[4] == 16 else [])
+            alts += [b[:8] + (b[8] | o,) for o in opts]
     for s in alts:
         if s not in out and feasible(prec, n, n, s):
             out.append(s)
@@ -124,7 +128,7 @@ HDR = """// {what} schedule and launch tables (the kernels are instantiated in
 // Column kernel: (Ny, W = columns per block, R, BY, BX = brick height/width, GF, GI, PP)
 // Schedule:      (Nx/2, Ny, BY, BX, row R, row G, col R, col GF, col GI, col W, PP, default)
 // PP: option bits (1 double-buffered LDS, 2 / 4 streaming column / row stores,
-// 8 streaming column-pass P, Q loads)
+// 8 streaming column-pass P, Q loads, 16 integrating-factor table at radix 16)
 #include "kernels_inst.hpp"
 
 namespace af {{

@@ -5,7 +5,7 @@
 // Column kernel: (Ny, W = columns per block, R, BY, BX = brick height/width, GF, GI, PP)
 // Schedule:      (Nx/2, Ny, BY, BX, row R, row G, col R, col GF, col GI, col W, PP, default)
 // PP: option bits (1 double-buffered LDS, 2 / 4 streaming column / row stores,
-// 8 streaming column-pass P, Q loads)
+// 8 streaming column-pass P, Q loads, 16 integrating-factor table at radix 16)
 #include "kernels_inst.hpp"
 
 namespace af {
@@ -200,11 +200,19 @@ extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 0>(const v
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 2>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 8>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 10>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 16>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 18>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 24>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 26>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 2>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 8>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 10>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 16>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 18>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 24>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 26>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 1, 8, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 1, 16, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 1, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
@@ -214,6 +222,10 @@ extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 0>(const v
 extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 2>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 8>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 10>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 16>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 18>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 24>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 26>(const void*, int, int, int, const void*, hipStream_t);
 namespace {
 using T = float;
 const RowCfg kRow[] = {
@@ -410,11 +422,19 @@ const ColCfg kCol[] = {
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 2>, 1, 1, 2},
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 8>, 1, 1, 8},
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 10>, 1, 1, 10},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 16>, 1, 1, 16},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 18>, 1, 1, 18},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 24>, 1, 1, 24},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 26>, 1, 1, 26},
     {2048, 2, 16, 4, 4, 256, launch_col<T, 2048, 2, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 2>, 1, 1, 2},
     {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 8>, 1, 1, 8},
     {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 10>, 1, 1, 10},
+    {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 16>, 1, 1, 16},
+    {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 18>, 1, 1, 18},
+    {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 24>, 1, 1, 24},
+    {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 26>, 1, 1, 26},
     {4096, 1, 8, 2, 2, 512, launch_col<T, 4096, 1, 8, 2, 2, 1, 1, 0>, 1, 1, 0},
     {4096, 1, 16, 2, 2, 256, launch_col<T, 4096, 1, 16, 2, 2, 1, 1, 0>, 1, 1, 0},
     {4096, 1, 16, 4, 2, 256, launch_col<T, 4096, 1, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
@@ -424,6 +444,10 @@ const ColCfg kCol[] = {
     {4096, 2, 16, 4, 4, 512, launch_col<T, 4096, 2, 16, 4, 4, 1, 1, 2>, 1, 1, 2},
     {4096, 2, 16, 4, 4, 512, launch_col<T, 4096, 2, 16, 4, 4, 1, 1, 8>, 1, 1, 8},
     {4096, 2, 16, 4, 4, 512, launch_col<T, 4096, 2, 16, 4, 4, 1, 1, 10>, 1, 1, 10},
+    {4096, 2, 16, 4, 4, 512, launch_col<T, 4096, 2, 16, 4, 4, 1, 1, 16>, 1, 1, 16},
+    {4096, 2, 16, 4, 4, 512, launch_col<T, 4096, 2, 16, 4, 4, 1, 1, 18>, 1, 1, 18},
+    {4096, 2, 16, 4, 4, 512, launch_col<T, 4096, 2, 16, 4, 4, 1, 1, 24>, 1, 1, 24},
+    {4096, 2, 16, 4, 4, 512, launch_col<T, 4096, 2, 16, 4, 4, 1, 1, 26>, 1, 1, 26},
 };
 const Schedule kSched[] = {
     {16, 32, 4, 2, 4, 3, 8, 2, 3, 2, 0, 1},
@@ -656,6 +680,14 @@ const Schedule kSched[] = {
     {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 10, 0},
     {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 12, 0},
     {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 14, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 16, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 18, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 20, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 22, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 24, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 26, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 28, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 30, 0},
     {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 2, 0},
     {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 4, 0},
     {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 6, 0},
@@ -663,6 +695,14 @@ const Schedule kSched[] = {
     {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 10, 0},
     {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 12, 0},
     {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 14, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 16, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 18, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 20, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 22, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 24, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 26, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 28, 0},
+    {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 30, 0},
     {1024, 4096, 4, 2, 8, 3, 16, 1, 1, 2, 0, 1},
     {2048, 32, 2, 2, 8, 1, 8, 2, 3, 2, 0, 1},
     {2048, 64, 2, 2, 8, 1, 8, 2, 3, 2, 0, 1},
@@ -685,6 +725,14 @@ const Schedule kSched[] = {
     {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 10, 0},
     {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 12, 0},
     {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 14, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 16, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 18, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 20, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 22, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 24, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 26, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 28, 0},
+    {2048, 4096, 4, 4, 16, 1, 16, 1, 1, 2, 30, 0},
 };
 }  // namespace
 

@@ -5,7 +5,7 @@
 // Column kernel: (Ny, W = columns per block, R, BY, BX = brick height/width, GF, GI, PP)
 // Schedule:      (Nx/2, Ny, BY, BX, row R, row G, col R, col GF, col GI, col W, PP, default)
 // PP: option bits (1 double-buffered LDS, 2 / 4 streaming column / row stores,
-// 8 streaming column-pass P, Q loads)
+// 8 streaming column-pass P, Q loads, 16 integrating-factor table at radix 16)
 #include "kernels_inst.hpp"
 
 namespace af {

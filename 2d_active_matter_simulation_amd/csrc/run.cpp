@@ -111,7 +111,8 @@ Geometry geometry(const af_params* p, bool batched) {
   g.off_desc = off; off += align256(sizeof(InstDesc<double>));
   g.off_pool = off; off += align256((size_t)2 * g.lmax * g.esz);
   g.off_ifac = off;
-  g.ifac = g.prec == 32 && g.cc && g.cc->r <= 8;   // the column pass reads the table (kTable)
+  // the column pass reads the table (kTable): FP32 at radix <= 8, or by option
+  g.ifac = g.prec == 32 && g.cc && (g.cc->r <= 8 || (g.cc->pp & kOptIfTable));
   if (g.ifac) off += align256(2 * g.nfield * sizeof(float));
   g.bytes = off;
   return g;

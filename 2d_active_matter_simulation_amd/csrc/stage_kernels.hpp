@@ -389,7 +389,8 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 // kernel keeps only the state it reads live (fewer registers, more waves).  GF = 2 / GI = 3: the forward (P, Q) and inverse
 // (v_x, v_y, omega) transforms share LDS exchanges; 1: one at a time.
 // PP, option bits: 1 = double-buffered LDS exchanges; 2 = streaming stores;
-// 8 = streaming loads of P, Q.
+// 8 = streaming loads of P, Q; 16 = FP32 integrating factors from the table
+// at radix 16 too.
 // ---------------------------------------------------------------------------
 template <class T, int NY, int W, int R, int BY, int BX, int GF, int GI, int PP, int stage>
 __global__ void __launch_bounds__(W * (NY / R))
@@ -493,11 +494,11 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 #pragma unroll
       for (int r = 0; r < R; ++r) av[r] = ld((stage == 1) ? d.wo : d.acc, r);
     };
-    // FP32 with radix <= 8: integrating factors from the run's table
-    // (FP64-evaluated, rounded once), loaded with omega_old; otherwise (FP64,
-    // or radix 16 where the block is latency bound and the extra loads cost
-    // more than the FP64 exp they save) evaluated in the update.
-    constexpr bool kTable = sizeof(T) == 4 && R <= 8;
+    // FP32 with radix <= 8 (or option 16): integrating factors from the run's
+    // table (FP64-evaluated, rounded once), loaded with omega_old; otherwise
+    // (FP64, or radix 16 in a latency-bound single run, where the extra loads
+    // cost more than the FP64 exp they save) evaluated in the update.
+    constexpr bool kTable = sizeof(T) == 4 && (R <= 8 || (PP & 16));
     float ef[2][R];
     auto load_wv = [&]() {
       if (stage == 2 || stage == 3) {

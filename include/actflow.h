@@ -122,8 +122,10 @@ typedef struct {
                              * block); 2 streaming (nt) column-pass stores;   *
                              * 4 streaming row-pass stores instead of device- *
                              * scope write-through (sc1); 8 streaming column- *
-                             * pass loads of P, Q.  2 and 4 act on FP32 runs  *
-                             * only (FP64 stores take the default policy).    */
+                             * pass loads of P, Q; 16 FP32 integrating        *
+                             * factors from a per-run table also at radix 16. *
+                             * 2, 4 and 16 act on FP32 runs only (FP64 stores *
+                             * take the default policy).                      */
 } af_schedule;
 int af_schedule_count(int precision, int nx, int ny);
 af_status af_schedule_get(int precision, int nx, int ny, int index, af_schedule* out);

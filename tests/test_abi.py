@@ -92,5 +92,5 @@ def test_schedule_struct_fields(af):
         for sc in af.schedules(prec, 1024):
             by, bx, rr, rg, cr, gf, gi, cw, opts = sc
             assert bx % cw == 0 and rr in (4, 8, 16) and cr in (4, 8, 16)
-            assert 0 <= opts < 16 and (prec == 32 or opts in (0, 1))
+            assert 0 <= opts < 32 and (prec == 32 or opts in (0, 1))
     assert any(sc[8] & 6 for sc in af.schedules(32, 1024))

@@ -62,3 +62,31 @@ def test_narrow_blocks_reduce_every_partial(af, orc):
     wo, mo = o.get_state()
     assert np.allclose(m, mo, rtol=1e-12)
     assert rel_l2(w, wo) <= 1e-12
+
+
+def test_factor_table_option_at_radix16(af):
+    """Option 16 (integrating factors read from the per-run FP32 table by a
+    radix-16 column pass) computes the step of the same schedule evaluating
+    them in the kernel: the table holds the same FP64-evaluated factors
+    rounded once, so the states agree to FP32 rounding."""
+    n = 2048
+    scheds = af.schedules(32, n)
+    pairs = [(i, scheds.index(sc[:8] + (sc[8] | 16,))) for i, sc in enumerate(scheds)
+             if sc[4] == 16 and not sc[8] & 16 and sc[:8] + (sc[8] | 16,) in scheds]
+    if not pairs:
+        pytest.skip("no radix-16 column schedule with the table option at 2048^2")
+    p = af.params(n, dt=5e-5, precision=32, **JW)
+    w0 = af.ic_bandlimited(n, seed=4, amp=2.0, kmax=30.0)
+    states = {}
+    try:
+        for i in {k for pr in pairs[:2] for k in pr}:
+            af.select_schedule(32, n, n, i)
+            run = af.Run(p)
+            run.set_state(w0, (0.1 * n * n, 0.0))
+            run.step(3)
+            states[i] = run.get_state()[0]
+            run.close()
+    finally:
+        af.select_schedule(32, n, n, -1)
+    for a, b in pairs[:2]:
+        assert rel_l2(states[b], states[a]) <= 1e-6, (scheds[a], scheds[b], rel_l2(states[b], states[a]))
