@@ -223,6 +223,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip FP64 / 2048^2 / 4096^2 lines")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble and batched legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the served-workload leg")
+    ap.add_argument("--grid", type=int, default=1024,
+                    help="grid of the single-run leg (default 1024, the headline; other sizes for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device-to-device copies of 256 MiB (the reference kernel for scaling "
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes), then exit")
@@ -244,9 +246,10 @@ def main():
     af = importlib.import_module("2d_active_matter_simulation_amd")
     af.lib()
 
-    head = run_workload(af, 1024, 32, args.steps, args.warmup)
+    N = args.grid
+    head = run_workload(af, N, 32, args.steps, args.warmup)
     name, alg, kms, achieved = roofline(af, head)
-    traffic = read_traffic("1024_fp32")
+    traffic = read_traffic(f"{N}_fp32")
     tr_bytes = None
     if traffic and f"{name}_bytes_per_launch" in traffic:
         tr_bytes = traffic[f"{name}_bytes_per_launch"]
@@ -261,7 +264,8 @@ def main():
                                       "frac": ach / HBM_PEAK_GBS,
                                       "step_model_frac": step_model_bytes(n, prec)
                                       / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
-        extra["fp32_over_fp64_1024"] = head["value"] / extra["1024_fp64"]["value"]
+        if N == 1024:
+            extra["fp32_over_fp64_1024"] = head["value"] / extra["1024_fp64"]["value"]
     out = {
         "metric": METRIC,
         "value": head["value"],
@@ -275,15 +279,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "single run 1024x1024, FP32 fields/FFTs, FP64 IF factors + reductions, dt=1e-4, "
+        "config": {"workload": f"single run {N}x{N}, FP32 fields/FFTs, FP64 IF factors + reductions, "
+                               f"dt={'1e-4' if N <= 1024 else '5e-5'}, "
                                "lambda0=3.5 alpha=-0.8 beta=0.01 Gamma0=-0.045 Gamma2=2.5e-5, band-limited IC "
                                "(seed 1, amp 1e-3, |k|<=10), per-step energy on",
-                   "grid": [1024, 1024], "instances": 1},
+                   "grid": [N, N], "instances": 1},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": tr_bytes,
                      "algorithmic_bytes_per_launch": alg, "avg_launch_ms": kms,
                      "events_over_steps": head["prof_steps"],
-                     "step_model_frac": step_model_bytes(1024) / (head["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                     "step_model_frac": step_model_bytes(N) / (head["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "extra": extra,
     }
     if not args.no_ensemble:
