@@ -95,6 +95,13 @@ def candidates(prec, n):
                  (4, 2, 16, 1, 8, 1, 1, 2, 1), (4, 2, 8, 1, 8, 1, 1, 2, 1)]
     else:
         alts += [(4, 2, 4, 3, 4, 2, 3, 2, 1), (4, 1, 8, 3, 8, 2, 3, 1, 1), (4, 2, 8, 1, 8, 1, 1, 1, 1)]
+    if prec == 32 and n in (512, 1024):
+        # one field per exchange on one side (stores of a finished field drain
+        # while the next one is transformed), with the cache options of the
+        # single-run winner
+        for rg, gf, gi in ((1, 2, 3), (3, 2, 1), (3, 1, 1), (1, 2, 1), (1, 1, 1)):
+            for o in (1, 3, 7, 11, 15):
+                alts.append((4, 2, 4, rg, 4, gf, gi, 2 if n == 1024 else 1, o))
     if prec == 32:
         for b in STORE_VARIANT_BASES.get(n, []):
             opts = list(range(2, 16, 2)) + ([16 + o for o in range(0, 16, 2)] if b[4] == 16 else [])

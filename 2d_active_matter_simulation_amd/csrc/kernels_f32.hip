@@ -37,9 +37,12 @@ extern template hipError_t launch_row<float, 256, 2, 8, 2, 1, 0>(const void*, in
 extern template hipError_t launch_row<float, 256, 2, 8, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 2, 8, 4, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 2, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 256, 4, 4, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 256, 4, 4, 2, 1, 5>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 4, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 4, 2, 3, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 4, 2, 3, 4>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 256, 4, 4, 2, 3, 5>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 8, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 256, 4, 8, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
@@ -57,6 +60,8 @@ extern template hipError_t launch_row<float, 512, 2, 8, 2, 1, 0>(const void*, in
 extern template hipError_t launch_row<float, 512, 2, 8, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 2, 8, 4, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 2, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 512, 4, 4, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 512, 4, 4, 2, 1, 5>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 4, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 4, 2, 3, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 512, 4, 4, 2, 3, 5>(const void*, int, int, const void*, hipStream_t);
@@ -124,11 +129,19 @@ extern template hipError_t launch_col<float, 256, 4, 8, 2, 4, 2, 3, 0>(const voi
 extern template hipError_t launch_col<float, 256, 4, 8, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 4, 8, 4, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 256, 4, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 1, 1, 1>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 1, 1, 3>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 1, 1, 11>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 1, 1>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 1, 3>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 1, 11>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 2>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 3>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 8>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 10>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 512, 1, 4, 4, 2, 2, 3, 11>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 8, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 8, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 512, 1, 8, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
@@ -162,6 +175,12 @@ extern template hipError_t launch_col<float, 1024, 1, 8, 4, 2, 2, 3, 0>(const vo
 extern template hipError_t launch_col<float, 1024, 1, 16, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 1, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 4, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 1, 1, 1>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 1, 1, 3>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 1, 1, 11>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 2, 1, 1>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 2, 1, 3>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 2, 1, 11>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 1024, 2, 4, 4, 2, 2, 3, 3>(const void*, int, int, int, const void*, hipStream_t);
@@ -257,9 +276,12 @@ const RowCfg kRow[] = {
     {256, 2, 8, 2, 64, launch_row<T, 256, 2, 8, 2, 3, 0>, 3, 0},
     {256, 2, 8, 4, 64, launch_row<T, 256, 2, 8, 4, 3, 0>, 3, 0},
     {256, 2, 16, 2, 32, launch_row<T, 256, 2, 16, 2, 1, 0>, 1, 0},
+    {256, 4, 4, 2, 256, launch_row<T, 256, 4, 4, 2, 1, 1>, 1, 1},
+    {256, 4, 4, 2, 256, launch_row<T, 256, 4, 4, 2, 1, 5>, 1, 5},
     {256, 4, 4, 2, 256, launch_row<T, 256, 4, 4, 2, 3, 0>, 3, 0},
     {256, 4, 4, 2, 256, launch_row<T, 256, 4, 4, 2, 3, 1>, 3, 1},
     {256, 4, 4, 2, 256, launch_row<T, 256, 4, 4, 2, 3, 4>, 3, 4},
+    {256, 4, 4, 2, 256, launch_row<T, 256, 4, 4, 2, 3, 5>, 3, 5},
     {256, 4, 8, 2, 128, launch_row<T, 256, 4, 8, 2, 1, 0>, 1, 0},
     {256, 4, 8, 2, 128, launch_row<T, 256, 4, 8, 2, 1, 1>, 1, 1},
     {256, 4, 8, 2, 128, launch_row<T, 256, 4, 8, 2, 3, 0>, 3, 0},
@@ -277,6 +299,8 @@ const RowCfg kRow[] = {
     {512, 2, 8, 2, 128, launch_row<T, 512, 2, 8, 2, 3, 0>, 3, 0},
     {512, 2, 8, 4, 128, launch_row<T, 512, 2, 8, 4, 3, 0>, 3, 0},
     {512, 2, 16, 2, 64, launch_row<T, 512, 2, 16, 2, 1, 0>, 1, 0},
+    {512, 4, 4, 2, 512, launch_row<T, 512, 4, 4, 2, 1, 1>, 1, 1},
+    {512, 4, 4, 2, 512, launch_row<T, 512, 4, 4, 2, 1, 5>, 1, 5},
     {512, 4, 4, 2, 512, launch_row<T, 512, 4, 4, 2, 3, 0>, 3, 0},
     {512, 4, 4, 2, 512, launch_row<T, 512, 4, 4, 2, 3, 1>, 3, 1},
     {512, 4, 4, 2, 512, launch_row<T, 512, 4, 4, 2, 3, 5>, 3, 5},
@@ -346,11 +370,19 @@ const ColCfg kCol[] = {
     {256, 4, 8, 4, 4, 128, launch_col<T, 256, 4, 8, 4, 4, 1, 1, 0>, 1, 1, 0},
     {256, 4, 8, 4, 4, 128, launch_col<T, 256, 4, 8, 4, 4, 2, 3, 0>, 2, 3, 0},
     {256, 4, 16, 4, 4, 64, launch_col<T, 256, 4, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
+    {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 1, 1, 1>, 1, 1, 1},
+    {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 1, 1, 3>, 1, 1, 3},
+    {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 1, 1, 11>, 1, 1, 11},
+    {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 1, 1>, 2, 1, 1},
+    {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 1, 3>, 2, 1, 3},
+    {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 1, 11>, 2, 1, 11},
     {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 0>, 2, 3, 0},
     {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 1>, 2, 3, 1},
     {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 2>, 2, 3, 2},
+    {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 3>, 2, 3, 3},
     {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 8>, 2, 3, 8},
     {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 10>, 2, 3, 10},
+    {512, 1, 4, 4, 2, 128, launch_col<T, 512, 1, 4, 4, 2, 2, 3, 11>, 2, 3, 11},
     {512, 1, 8, 2, 2, 64, launch_col<T, 512, 1, 8, 2, 2, 1, 1, 0>, 1, 1, 0},
     {512, 1, 8, 4, 2, 64, launch_col<T, 512, 1, 8, 4, 2, 1, 1, 0>, 1, 1, 0},
     {512, 1, 8, 4, 2, 64, launch_col<T, 512, 1, 8, 4, 2, 2, 3, 0>, 2, 3, 0},
@@ -384,6 +416,12 @@ const ColCfg kCol[] = {
     {1024, 1, 16, 2, 2, 64, launch_col<T, 1024, 1, 16, 2, 2, 1, 1, 0>, 1, 1, 0},
     {1024, 1, 16, 4, 2, 64, launch_col<T, 1024, 1, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
     {1024, 2, 4, 2, 2, 512, launch_col<T, 1024, 2, 4, 2, 2, 1, 1, 0>, 1, 1, 0},
+    {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 1, 1, 1>, 1, 1, 1},
+    {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 1, 1, 3>, 1, 1, 3},
+    {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 1, 1, 11>, 1, 1, 11},
+    {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 2, 1, 1>, 2, 1, 1},
+    {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 2, 1, 3>, 2, 1, 3},
+    {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 2, 1, 11>, 2, 1, 11},
     {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 2, 3, 0>, 2, 3, 0},
     {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 2, 3, 1>, 2, 3, 1},
     {1024, 2, 4, 4, 2, 512, launch_col<T, 1024, 2, 4, 4, 2, 2, 3, 3>, 2, 3, 3},
@@ -568,6 +606,31 @@ const Schedule kSched[] = {
     {256, 512, 4, 2, 8, 3, 8, 2, 3, 2, 1, 0},
     {256, 512, 4, 2, 16, 1, 8, 1, 1, 2, 1, 0},
     {256, 512, 4, 2, 8, 1, 8, 1, 1, 2, 1, 0},
+    {256, 512, 4, 2, 4, 1, 4, 2, 3, 1, 1, 0},
+    {256, 512, 4, 2, 4, 1, 4, 2, 3, 1, 3, 0},
+    {256, 512, 4, 2, 4, 1, 4, 2, 3, 1, 7, 0},
+    {256, 512, 4, 2, 4, 1, 4, 2, 3, 1, 11, 0},
+    {256, 512, 4, 2, 4, 1, 4, 2, 3, 1, 15, 0},
+    {256, 512, 4, 2, 4, 3, 4, 2, 1, 1, 1, 0},
+    {256, 512, 4, 2, 4, 3, 4, 2, 1, 1, 3, 0},
+    {256, 512, 4, 2, 4, 3, 4, 2, 1, 1, 7, 0},
+    {256, 512, 4, 2, 4, 3, 4, 2, 1, 1, 11, 0},
+    {256, 512, 4, 2, 4, 3, 4, 2, 1, 1, 15, 0},
+    {256, 512, 4, 2, 4, 3, 4, 1, 1, 1, 1, 0},
+    {256, 512, 4, 2, 4, 3, 4, 1, 1, 1, 3, 0},
+    {256, 512, 4, 2, 4, 3, 4, 1, 1, 1, 7, 0},
+    {256, 512, 4, 2, 4, 3, 4, 1, 1, 1, 11, 0},
+    {256, 512, 4, 2, 4, 3, 4, 1, 1, 1, 15, 0},
+    {256, 512, 4, 2, 4, 1, 4, 2, 1, 1, 1, 0},
+    {256, 512, 4, 2, 4, 1, 4, 2, 1, 1, 3, 0},
+    {256, 512, 4, 2, 4, 1, 4, 2, 1, 1, 7, 0},
+    {256, 512, 4, 2, 4, 1, 4, 2, 1, 1, 11, 0},
+    {256, 512, 4, 2, 4, 1, 4, 2, 1, 1, 15, 0},
+    {256, 512, 4, 2, 4, 1, 4, 1, 1, 1, 1, 0},
+    {256, 512, 4, 2, 4, 1, 4, 1, 1, 1, 3, 0},
+    {256, 512, 4, 2, 4, 1, 4, 1, 1, 1, 7, 0},
+    {256, 512, 4, 2, 4, 1, 4, 1, 1, 1, 11, 0},
+    {256, 512, 4, 2, 4, 1, 4, 1, 1, 1, 15, 0},
     {256, 512, 4, 2, 4, 3, 4, 2, 3, 1, 2, 0},
     {256, 512, 4, 2, 4, 3, 4, 2, 3, 1, 4, 0},
     {256, 512, 4, 2, 4, 3, 4, 2, 3, 1, 6, 0},
@@ -628,6 +691,31 @@ const Schedule kSched[] = {
     {512, 1024, 4, 2, 8, 3, 8, 2, 3, 2, 1, 0},
     {512, 1024, 4, 2, 16, 1, 8, 1, 1, 2, 1, 0},
     {512, 1024, 4, 2, 8, 1, 8, 1, 1, 2, 1, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 2, 3, 2, 1, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 2, 3, 2, 3, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 2, 3, 2, 7, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 2, 3, 2, 11, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 2, 3, 2, 15, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 2, 1, 2, 1, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 2, 1, 2, 3, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 2, 1, 2, 7, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 2, 1, 2, 11, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 2, 1, 2, 15, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 1, 1, 2, 1, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 1, 1, 2, 3, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 1, 1, 2, 7, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 1, 1, 2, 11, 0},
+    {512, 1024, 4, 2, 4, 3, 4, 1, 1, 2, 15, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 2, 1, 2, 1, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 2, 1, 2, 3, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 2, 1, 2, 7, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 2, 1, 2, 11, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 2, 1, 2, 15, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 1, 1, 2, 1, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 1, 1, 2, 3, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 1, 1, 2, 7, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 1, 1, 2, 11, 0},
+    {512, 1024, 4, 2, 4, 1, 4, 1, 1, 2, 15, 0},
     {512, 1024, 4, 2, 4, 3, 4, 2, 3, 2, 3, 0},
     {512, 1024, 4, 2, 4, 3, 4, 2, 3, 2, 5, 0},
     {512, 1024, 4, 2, 4, 3, 4, 2, 3, 2, 7, 0},
