@@ -92,7 +92,8 @@ def main():
                 if ok and (best is None or ms < best[0]):
                     best = (ms, sc)
             if best:
-                lines.append(f"{prec} {n} {n} " + " ".join(map(str, best[1])) + f"   # {best[0]:.4f}")
+                note = f" ({args.instances} instances)" if args.instances > 1 else ""
+                lines.append(f"{prec} {n} {n} " + " ".join(map(str, best[1])) + f"   # {best[0]:.4f}{note}")
                 print(f"  -> best {best[1]} {best[0]:.4f} ms/step", flush=True)
     # keep the existing table's lines for the (precision, size) pairs not tuned now
     tuned = {tuple(ln.split()[:3]) for ln in lines[1:]}
