@@ -1,6 +1,8 @@
 """Ensemble engine: runs of mixed size and parameters stepped in shared
 launches give bitwise the same results as each run stepped alone; admission
 respects the HBM budget."""
+import os
+
 import numpy as np
 import pytest
 
@@ -22,10 +24,27 @@ def _cases(af):
     return out
 
 
+def _engine_table_schedule(af, prec, n):
+    """The schedule the engine's tuned table (schedules_batched.txt next to
+    the library) names for an n x n grid, or None (the engine then takes the
+    single-run choice)."""
+    path = os.path.join(os.path.dirname(af.LIB_PATH), "schedules_batched.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        v = line.split("#")[0].split()
+        if len(v) >= 10 and (int(v[0]), int(v[1]), int(v[2])) == (prec, n, n):
+            sc = tuple(int(x) for x in v[3:])
+            if len(sc) == 7:
+                sc += (sc[1],)
+            return sc + (0,) if len(sc) == 8 else sc
+    return None
+
+
 def test_alone_equals_mix_bitwise(af):
     """Each run stepped in a mix equals the same run stepped alone in an
-    engine, bitwise; and equals the single-run API (bitwise at these sizes,
-    where both use the same schedule)."""
+    engine, bitwise; and equals the single-run API, bitwise where the engine
+    and single runs use the same schedule, else to FP rounding."""
     cases = _cases(af)
     eng = af.Engine(1 << 34)
     slots = [eng.add(p, w, M) for p, w, M in cases]
@@ -45,8 +64,14 @@ def test_alone_equals_mix_bitwise(af):
         run.set_state(w, M)
         e_run = run.step(30, energy=True)
         wr, mr = run.get_state()
-        assert np.array_equal(wr, wm) and np.array_equal(mr, mm), s
-        assert np.array_equal(eng.energy(s, 30), e_run)
+        esched = _engine_table_schedule(af, p.precision, p.nx)
+        if esched is None or esched == af.active_schedule(p.precision, p.nx):
+            assert np.array_equal(wr, wm) and np.array_equal(mr, mm), s
+            assert np.array_equal(eng.energy(s, 30), e_run)
+        else:
+            tol = 1e-5 if p.precision == 32 else 1e-12
+            assert np.linalg.norm(wr - wm) <= tol * np.linalg.norm(wr), s
+            assert np.allclose(mr, mm, rtol=tol) and np.allclose(eng.energy(s, 30), e_run, rtol=tol), s
         run.close()
     eng.close()
 
