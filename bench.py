@@ -209,9 +209,29 @@ def cpu_baseline(n=1024, prec=32, target_s=10.0):
     t0 = time.perf_counter()
     o.step(steps, energy=True)
     wall = time.perf_counter() - t0
-    return {"value": n * n * steps / wall, "unit": "grid-pt-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps of the {n}^2 FP{prec} oracle after 1 warm-up step "
-                      f"({wall:.1f} s, {threads} OpenMP threads)"}
+    out = {"value": n * n * steps / wall, "unit": "grid-pt-steps/s", "cores": threads, "kind": "port",
+           "sample": f"{steps} steps of the {n}^2 FP{prec} oracle after 1 warm-up step "
+                     f"({wall:.1f} s, {threads} OpenMP threads)"}
+    # the same on one thread, a few steps (~target_s / 2)
+    o1 = Oracle(n, dt=1e-4, lam=PHYS["lam"], beta=PHYS["beta"], c0=-PHYS["alpha"], c2=-PHYS["gamma0"],
+                c4=-PHYS["gamma2"], precision=prec, nthreads=1)
+    o1.set_state(af.ic_bandlimited(n, seed=1, amp=1e-3, kmax=10.0))
+    t0 = time.perf_counter()
+    o1.step(1)
+    first1 = time.perf_counter() - t0
+    steps1 = max(2, int(target_s / 2 / max(first1, 1e-3)))
+    t0 = time.perf_counter()
+    o1.step(steps1, energy=True)
+    wall1 = time.perf_counter() - t0
+    out["single_thread"] = {"value": n * n * steps1 / wall1, "cores": 1, "sample": f"{steps1} steps ({wall1:.1f} s)"}
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    out["host"] = {"cpu_count": os.cpu_count(), "model": model}
+    return out
 
 
 def main():
