@@ -200,6 +200,29 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red /* LDS, N
   }
 }
 
+// First half of a deterministic block sum that needs no barrier of its own:
+// each wave folds its lanes (butterfly) and lane 0 parks the wave sums in
+// red[wave * NV + k]; after a later barrier, lds_wave_sum adds them in wave
+// order.
+template <int NV, int NT>
+__device__ __forceinline__ void wave_fold_to_lds(double (&v)[NV], double* red /* LDS, NV * waves */) {
+  constexpr int WW = NT < 64 ? NT : 64;   // lanes of a (possibly partial) wave
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int off = WW / 2; off >= 1; off >>= 1) v[k] += __shfl_xor(v[k], off, WW);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[(threadIdx.x >> 6) * NV + k] = v[k];
+}
+template <int NV, int NT>
+__device__ __forceinline__ double lds_wave_sum(const double* red, int k) {
+  double s = red[k];
+#pragma unroll
+  for (int w = 1; w < (NT + 63) / 64; ++w) s += red[w * NV + k];
+  return s;
+}
+
 // ---------------------------------------------------------------------------
 // Row pass.  Block = RB rows (one brick row) x TR = NX2/R lanes per row.
 // G = 3: the three C2R transforms share their LDS exchanges (and the two
@@ -404,7 +427,8 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   constexpr int kPqCP = (PP & 8) ? kCacheNt : kCacheDefault;
   __shared__ cx<T> lds_mem[View::kElems];
   __shared__ cx<T> tw_y[TwLayout<NY, R>::kAlloc];
-  __shared__ double mean_in[7];   // tile 0: sum|u|^2, sum Cx, sum Cy, M, M_old
+  __shared__ double mean_in[4];   // tile 0: M, M_old
+  __shared__ double mean_red[3 * ((NT + 63) / 64)];   // tile 0: wave sums of the row partials
   __shared__ long long nstep_in;
 
   // a tile reads BY x W elements of each BY-row band: how many column tiles
@@ -436,8 +460,6 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 
   const double dt = d.dt, dt2 = d.dt2, dt3 = d.dt3, dt6 = d.dt6;
   const bool col0 = (x == 0);
-  constexpr int RW = NT < 64 ? NT : 64;           // lanes of the first (possibly partial) wave
-  const bool mean_wave = (tile == 0 && t < RW);   // owns the k = 0 mode and the mean ODE
   const double kxd = d.tkx * (double)x;
   const double kx2 = kxd * kxd;
   const int ymax = mask_ymax(x, d.nx, d.ny, d.mask_mode);
@@ -455,36 +477,30 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     __syncthreads();
   } else {
     cx<T> pq[2][R], wv[R], av[R];
+    // Tile 0: the row-block partials of the mean ODE, spread over every lane
+    // of the block; each wave folds them into LDS, and the ODE adds the wave
+    // sums in wave order (a fixed order).  The
+    // mean state waits in LDS too, so nothing stays in registers across the
+    // transforms and no lane holds the block up before its first barrier.
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       pq[0][r] = BufField<cx<T>>(d.pm).template ld<kPqCP>(vb, (unsigned)r * rstride);
       pq[1][r] = BufField<cx<T>>(d.qm).template ld<kPqCP>(vb, (unsigned)r * rstride);
     }
-    // Tile 0: row-block partials for the mean ODE (summed in a fixed order)
-    // and the mean state, parked in LDS until the ODE so that they occupy no
-    // registers across the transforms.
-    if (mean_wave) {
-      double s0 = 0, s1 = 0, s2 = 0;
+    if (tile == 0) {
+      double ps[3] = {0.0, 0.0, 0.0};
       const gptr<double> pp = glob(d.partials);
-      for (int i = t; i < d.row_tiles; i += RW) {
-        s0 += pp[(unsigned)i * 4 + 0];
-        s1 += pp[(unsigned)i * 4 + 1];
-        s2 += pp[(unsigned)i * 4 + 2];
+      for (int i = t; i < d.row_tiles; i += NT) {
+        ps[0] += pp[(unsigned)i * 4 + 0];
+        ps[1] += pp[(unsigned)i * 4 + 1];
+        ps[2] += pp[(unsigned)i * 4 + 2];
       }
-#pragma unroll
-      for (int off = RW / 2; off >= 1; off >>= 1) {
-        s0 += __shfl_xor(s0, off, RW);
-        s1 += __shfl_xor(s1, off, RW);
-        s2 += __shfl_xor(s2, off, RW);
-      }
+      wave_fold_to_lds<3, NT>(ps, mean_red);
       if (t == 0) {
-        mean_in[0] = s0;
-        mean_in[1] = s1;
-        mean_in[2] = s2;
-        mean_in[3] = d.mean->M[0];
-        mean_in[4] = d.mean->M[1];
-        mean_in[5] = d.mean->Mo[0];
-        mean_in[6] = d.mean->Mo[1];
+        mean_in[0] = d.mean->M[0];
+        mean_in[1] = d.mean->M[1];
+        mean_in[2] = d.mean->Mo[0];
+        mean_in[3] = d.mean->Mo[1];
         nstep_in = d.mean->nstep;
       }
     }
@@ -608,10 +624,10 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     // Mean-velocity ODE (host updates of .cu:729-821), FP64, tile 0 only.
     if (tile == 0 && t == 0) {
       MeanState* ms = d.mean;
-      const double s0 = mean_in[0];
+      const double s0 = lds_wave_sum<3, NT>(mean_red, 0);
       const long long nstep = nstep_in;
-      const double g[2] = {-d.beta * mean_in[1], -d.beta * mean_in[2]};
-      double M[2] = {mean_in[3], mean_in[4]}, Mo[2] = {mean_in[5], mean_in[6]}, mm[2];
+      const double g[2] = {-d.beta * lds_wave_sum<3, NT>(mean_red, 1), -d.beta * lds_wave_sum<3, NT>(mean_red, 2)};
+      double M[2] = {mean_in[0], mean_in[1]}, Mo[2] = {mean_in[2], mean_in[3]}, mm[2];
       if (stage == 1) {
         d.energy[nstep % d.ring_cap] = s0 * d.invN * 0.5;
         Mo[0] = M[0];
