@@ -259,8 +259,10 @@ struct af_run {
   Geometry g;
   hipStream_t stream = nullptr;
   unsigned char* base = nullptr;
-  hipGraph_t graph = nullptr;
+  hipGraph_t graph = nullptr;       // one step
   hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph_n = nullptr;     // kGraphSteps steps
+  hipGraphExec_t gexec_n = nullptr;
   bool use_graph = true;
   bool graph_failed = false;
   bool ready = false;
@@ -341,28 +343,45 @@ af_status af_run_get_state(af_run* r, double* omega_hat, double M[2]) {
   return store_state(r->g, r->base, omega_hat, M, r->stream);
 }
 
-static hipError_t run_one_step(af_run* r) {
+// Steps per multi-step graph: replaying one graph of 16 steps instead of 16
+// one-step graphs removes the gaps between graph launches.
+constexpr int kGraphSteps = 16;
+
+// Captures `steps` steps into (graph, exec); false (and nothing kept) on failure.
+static bool capture_steps(af_run* r, int steps, hipGraph_t* graph, hipGraphExec_t* exec) {
   const void* desc = r->base + r->g.off_desc;
   const void* pool = r->base + r->g.off_pool;
-  if (r->use_graph && !r->graph_failed) {
-    if (!r->gexec) {
-      hipError_t e = hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal);
-      if (e == hipSuccess) {
-        hipError_t le = launch_step(r->g, desc, 1, pool, r->stream);
-        e = hipStreamEndCapture(r->stream, &r->graph);
-        if (le == hipSuccess && e == hipSuccess) e = hipGraphInstantiate(&r->gexec, r->graph, nullptr, nullptr, 0);
-        else if (e == hipSuccess) e = le;
-      }
-      if (e != hipSuccess || !r->gexec) {
-        (void)hipGetLastError();
-        r->graph_failed = true;
-        if (r->graph) { (void)hipGraphDestroy(r->graph); r->graph = nullptr; }
-        return launch_step(r->g, desc, 1, pool, r->stream);
-      }
-    }
-    return hipGraphLaunch(r->gexec, r->stream);
+  hipError_t e = hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    hipError_t le = hipSuccess;
+    for (int i = 0; i < steps && le == hipSuccess; ++i) le = launch_step(r->g, desc, 1, pool, r->stream);
+    e = hipStreamEndCapture(r->stream, graph);
+    if (le == hipSuccess && e == hipSuccess) e = hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0);
+    else if (e == hipSuccess) e = le;
   }
-  return launch_step(r->g, desc, 1, pool, r->stream);
+  if (e != hipSuccess || !*exec) {
+    (void)hipGetLastError();
+    if (*graph) { (void)hipGraphDestroy(*graph); *graph = nullptr; }
+    *exec = nullptr;
+    return false;
+  }
+  return true;
+}
+
+// Enqueues `steps` (1 or kGraphSteps) steps: as a captured graph, or as plain
+// launches when graphs are off or could not be captured.
+static hipError_t run_steps(af_run* r, int steps) {
+  if (r->use_graph && !r->graph_failed) {
+    hipGraph_t* g = steps == 1 ? &r->graph : &r->graph_n;
+    hipGraphExec_t* x = steps == 1 ? &r->gexec : &r->gexec_n;
+    if (*x || capture_steps(r, steps, g, x)) return hipGraphLaunch(*x, r->stream);
+    r->graph_failed = true;
+  }
+  const void* desc = r->base + r->g.off_desc;
+  const void* pool = r->base + r->g.off_pool;
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < steps && e == hipSuccess; ++i) e = launch_step(r->g, desc, 1, pool, r->stream);
+  return e;
 }
 
 af_status af_run_step(af_run* r, int nsteps, double* energy, long long* nan_step) {
@@ -372,9 +391,11 @@ af_status af_run_step(af_run* r, int nsteps, double* energy, long long* nan_step
   int done = 0;
   while (done < nsteps) {
     const int k = (nsteps - done) < kRingCap ? (nsteps - done) : kRingCap;
-    for (int i = 0; i < k; ++i) {
-      hipError_t e = run_one_step(r);
+    for (int i = 0; i < k;) {
+      const int n = (k - i >= kGraphSteps) ? kGraphSteps : 1;
+      hipError_t e = run_steps(r, n);
       if (e != hipSuccess) return hip_fail(e, "step launch");
+      i += n;
     }
     MeanState ms;
     hipError_t e = hipMemcpyAsync(&ms, r->base + r->g.off_mean, sizeof(ms), hipMemcpyDeviceToHost, r->stream);
@@ -496,6 +517,8 @@ void af_run_destroy(af_run* r) {
   if (r->stream) (void)hipStreamSynchronize(r->stream);
   if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
   if (r->graph) (void)hipGraphDestroy(r->graph);
+  if (r->gexec_n) (void)hipGraphExecDestroy(r->gexec_n);
+  if (r->graph_n) (void)hipGraphDestroy(r->graph_n);
   if (r->base) (void)hipFree(r->base);
   if (r->stream) (void)hipStreamDestroy(r->stream);
   delete r;
