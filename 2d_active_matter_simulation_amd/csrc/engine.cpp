@@ -2,11 +2,13 @@
 // device.  Runs of the same (precision, Nx, Ny) form a size class; one step of
 // a class is the same 8 launches as a single run, each launch covering every
 // run of the class (blockIdx -> (run, tile) through an array of per-run
-// descriptors).  Classes are independent, so each gets its own stream and the
-// step of all classes is captured as one hipGraph that forks into a branch per
-// class and joins at the end: different sizes execute concurrently, no block
-// ever waits on another, and each run's arithmetic is exactly that of the run
-// stepped alone (same kernels, same per-run data, no cross-run reductions).
+// descriptors).  Classes are independent, so each gets its own stream and
+// the steps of all classes are captured as one hipGraph that forks into a
+// branch per class and joins at the end (one graph of one step, one of
+// kGraphSteps steps in which each class runs its steps back to back):
+// different sizes execute concurrently, no block ever waits on another, and
+// each run's arithmetic is exactly that of the run stepped alone (same
+// kernels, same per-run data, no cross-run reductions).
 //
 // Admission is by an HBM budget on the bytes of run state the engine holds.
 // No reference counterpart: the reference runs one grid per process.
@@ -21,11 +23,18 @@
 
 using namespace af;
 
+// Steps per multi-step graph (replaying one graph of 16 steps instead of 16
+// one-step graphs removes the gaps between graph launches and the per-step
+// joins of the classes).
+constexpr int kGraphSteps = 16;
+
 static void drop_graph(af_engine* e) {
-  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
-  if (e->graph) (void)hipGraphDestroy(e->graph);
-  e->gexec = nullptr;
-  e->graph = nullptr;
+  for (int i = 0; i < 2; ++i) {
+    if (e->gexec[i]) (void)hipGraphExecDestroy(e->gexec[i]);
+    if (e->graph[i]) (void)hipGraphDestroy(e->graph[i]);
+    e->gexec[i] = nullptr;
+    e->graph[i] = nullptr;
+  }
   e->graph_slots.clear();
 }
 
@@ -116,7 +125,10 @@ static af_status build_descs(af_engine* e, const std::vector<int>& list,
   return AF_OK;
 }
 
-static hipError_t enqueue_step(af_engine* e, const std::vector<std::pair<ClassKey, std::vector<int>>>& groups) {
+// `steps` steps of every class in `groups`: fork from the main stream, each
+// class's steps back to back on its own stream, join.
+static hipError_t enqueue_steps(af_engine* e, const std::vector<std::pair<ClassKey, std::vector<int>>>& groups,
+                                int steps) {
   // fork: every class stream waits on the main stream, joins back at the end
   if (e->events.size() < groups.size() + 1) {
     for (size_t i = e->events.size(); i < groups.size() + 1; ++i) {
@@ -132,9 +144,10 @@ static hipError_t enqueue_step(af_engine* e, const std::vector<std::pair<ClassKe
     Class& c = e->classes[groups[i].first];
     const Slot& s0 = e->slots[groups[i].second[0]];
     if ((he = hipStreamWaitEvent(c.stream, e->events[0], 0)) != hipSuccess) return he;
-    if ((he = launch_step(s0.g, c.descs, (int)groups[i].second.size(), s0.base + s0.g.off_pool, c.stream)) !=
-        hipSuccess)
-      return he;
+    for (int k = 0; k < steps; ++k)
+      if ((he = launch_step(s0.g, c.descs, (int)groups[i].second.size(), s0.base + s0.g.off_pool, c.stream)) !=
+          hipSuccess)
+        return he;
     if ((he = hipEventRecord(e->events[i + 1], c.stream)) != hipSuccess) return he;
     if ((he = hipStreamWaitEvent(e->stream, e->events[i + 1], 0)) != hipSuccess) return he;
   }
@@ -150,30 +163,34 @@ af_status af_engine_step_slots(af_engine* e, const int* slots, int nslots, int n
     if (id < 0 || id >= (int)e->slots.size() || !e->slots[id].used) return fail(AF_EINVAL, "bad slot in list");
   if (list.empty() || nsteps == 0) return AF_OK;
   std::vector<std::pair<ClassKey, std::vector<int>>> groups;
-  if (list != e->graph_slots || !e->gexec) {
+  if (list != e->graph_slots || !e->gexec[0]) {
     drop_graph(e);
     af_status st = build_descs(e, list, groups);
     if (st != AF_OK) return st;
-    hipError_t he = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
-    if (he == hipSuccess) {
-      hipError_t le = enqueue_step(e, groups);
-      he = hipStreamEndCapture(e->stream, &e->graph);
-      if (le != hipSuccess) he = le;
-      if (he == hipSuccess) he = hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0);
-    }
-    if (he != hipSuccess) {
-      (void)hipGetLastError();
-      drop_graph(e);
-      return hip_fail(he, "engine graph");
+    for (int gi = 0; gi < 2; ++gi) {
+      hipError_t he = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+      if (he == hipSuccess) {
+        hipError_t le = enqueue_steps(e, groups, gi == 0 ? 1 : kGraphSteps);
+        he = hipStreamEndCapture(e->stream, &e->graph[gi]);
+        if (le != hipSuccess) he = le;
+        if (he == hipSuccess) he = hipGraphInstantiate(&e->gexec[gi], e->graph[gi], nullptr, nullptr, 0);
+      }
+      if (he != hipSuccess) {
+        (void)hipGetLastError();
+        drop_graph(e);
+        return hip_fail(he, "engine graph");
+      }
     }
     e->graph_slots = list;
   }
   int done = 0;
   while (done < nsteps) {
     const int k = std::min(nsteps - done, kRingCap);
-    for (int i = 0; i < k; ++i) {
-      hipError_t he = hipGraphLaunch(e->gexec, e->stream);
+    for (int i = 0; i < k;) {
+      const int n = (k - i >= kGraphSteps) ? kGraphSteps : 1;
+      hipError_t he = hipGraphLaunch(e->gexec[n == 1 ? 0 : 1], e->stream);
       if (he != hipSuccess) return hip_fail(he, "engine step");
+      i += n;
     }
     hipError_t he = hipStreamSynchronize(e->stream);
     if (he != hipSuccess) return hip_fail(he, "engine sync");

@@ -32,9 +32,9 @@ struct af_engine {
   std::vector<af::Slot> slots;
   std::map<af::ClassKey, af::Class> classes;
   hipStream_t stream = nullptr;
-  // cached graph for the last stepped slot list
+  // cached graphs (one step, kGraphSteps steps) for the last stepped slot list
   std::vector<int> graph_slots;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<hipEvent_t> events;
 };
