@@ -251,7 +251,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(instances shard across GPUs without communication; see DESIGN.md).", file=sys.stderr)
+              "(instances would shard across GPUs without communication).", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         import torch
