@@ -167,6 +167,12 @@ struct FftMulti {
   static constexpr int RL = L / ipow(R, NP);  // trailing radix (1 if L is a power of R)
   static constexpr int S = L / R;             // lanes per transform
   static constexpr int ESZ = (int)sizeof(cx<T>);
+  // LDS set (0 or 1) the transform's final loads read, -1 if it has no
+  // exchange; its first stores (after pass 0) always go to set 0.  With two
+  // sets (BUF != 0) a caller may skip the barrier before the next transform
+  // when kLastLoadSet == 1: the next one's first stores cannot overwrite
+  // data a lagging lane still loads.
+  static constexpr int kLastLoadSet = (RL > 1) ? (NP - 1) % 2 : (NP >= 2 ? (NP - 2) % 2 : -1);
   static constexpr int AREA = lds_area_elems<L, W, R>();
   static __device__ __forceinline__ int pad(int i, int Ns) { return lds_pad<W, R, ESZ>(i, Ns); }
 

@@ -132,9 +132,14 @@ struct LdsView {
   static constexpr int kElems = (PP ? 2 : 1) * NFM * lds_area_elems<L, W, R>();
   cx<T>* base;
   int c;
-  // unpadded access for the even/odd pair exchange of the real transforms
-  __device__ __forceinline__ cx<T> load_plain(int f, int p) const { return base[f * lds_area_elems<L, W, R>() + p * W + c]; }
-  __device__ __forceinline__ void store_plain(int f, int p, cx<T> v) { base[f * lds_area_elems<L, W, R>() + p * W + c] = v; }
+  // unpadded access for the even/odd pair exchange of the real transforms,
+  // in set `set` (0 or 1; one set only without PP)
+  __device__ __forceinline__ cx<T> load_plain(int f, int p, int set = 0) const {
+    return base[set * kBuf + f * lds_area_elems<L, W, R>() + p * W + c];
+  }
+  __device__ __forceinline__ void store_plain(int f, int p, cx<T> v, int set = 0) {
+    base[set * kBuf + f * lds_area_elems<L, W, R>() + p * W + c] = v;
+  }
 };
 
 // Twiddle pool in HBM: tw[L + m] = exp(-2 pi i m / L) for every power of two L.
@@ -279,6 +284,19 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   stage_fft_twiddles<T, NT, NX2, R>(tw_h, tw_of(pool, NX2));
   stage_twiddles<T, NT>(tw_n, tw_of(pool, 2 * NX2), NX2);
 
+  // With two LDS sets (PP) the grouped path puts the pair exchanges in the
+  // set the neighbouring transform passes do not touch at that moment, which
+  // drops three barriers: the one between the pre-exchange loads and the
+  // first C2R stores (those go to set 0, the exchange uses set 1), the one
+  // before the R2C transforms when the C2R's last loads read set 1, and the
+  // one before the post-exchange stores (into the set the R2C's last loads
+  // did not read; every earlier load of it finished before their barrier).
+  constexpr bool kDbl = View::kBuf != 0;
+  constexpr int kLastC2R = FftMulti<NX2, R, +1, 3, RB, T, View::kBuf>::kLastLoadSet;
+  constexpr int kLastR2C = FftMulti<NX2, R, -1, 2, RB, T, View::kBuf>::kLastLoadSet;
+  constexpr int kPreSet = (kDbl && G == 3) ? 1 : 0;
+  constexpr int kPostSet = (kDbl && G == 3 && kLastR2C >= 0) ? 1 - kLastR2C : 0;
+
   // Row C2R of a half spectrum X[0..NX2) (X[NX2] = 0, Im X[0] ignored):
   // Z[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/N} (X[k] - conj X[M-k]),
   // then inverse length-M transform -> (x[2n], x[2n+1]).
@@ -291,7 +309,7 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
         xk.im = (T)0;
         xr = cx<T>{(T)0, (T)0};
       } else {
-        xr = cconj(lds.load_plain(f, NX2 - k));
+        xr = cconj(lds.load_plain(f, NX2 - k, kPreSet));
       }
       const cx<T> s = xk + xr, dd = xk - xr;
       q[r] = s + cmuli(cmulc(dd, tw_n[k]));
@@ -301,11 +319,11 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 #pragma unroll
     for (int f = 0; f < 3; ++f)
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds.store_plain(f, j + r * TR, f3[f][r]);
+      for (int r = 0; r < R; ++r) lds.store_plain(f, j + r * TR, f3[f][r], kPreSet);
     __syncthreads();
 #pragma unroll
     for (int f = 0; f < 3; ++f) c2r_pre(f3[f], f);
-    __syncthreads();
+    if constexpr (!kDbl) __syncthreads();
     fft_multi<NX2, R, +1, 3>(f3, j, lds, tw_h);
   } else {
 #pragma unroll
@@ -355,18 +373,18 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int k = j + r * TR;
-      const cx<T> zk = q[r], zm = cconj(lds.load_plain(f, (NX2 - k) & (NX2 - 1)));
+      const cx<T> zk = q[r], zm = cconj(lds.load_plain(f, (NX2 - k) & (NX2 - 1), kPostSet));
       q[r] = (zk + zm) - cmul(cmuli(zk - zm), tw_n[k]);
     }
   };
   if constexpr (GR == 2) {
-    __syncthreads();
+    if constexpr (!(kDbl && kLastC2R == 1)) __syncthreads();
     fft_multi<NX2, R, -1, 2>(f2, j, lds, tw_h);
-    __syncthreads();
+    if constexpr (!kDbl || kLastR2C < 0) __syncthreads();
 #pragma unroll
     for (int f = 0; f < 2; ++f)
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds.store_plain(f, j + r * TR, f2[f][r]);
+      for (int r = 0; r < R; ++r) lds.store_plain(f, j + r * TR, f2[f][r], kPostSet);
     __syncthreads();
 #pragma unroll
     for (int f = 0; f < 2; ++f) r2c_post(f2[f], f);
@@ -424,6 +442,11 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   static_assert((GF == 1 || GF == 2) && (GI == 1 || GI == 3), "field groups");
   using View = LdsView<T, NY, W, R, GMAX, PP & 1>;
   constexpr int kCP = (PP & 2) ? kCacheNt : kCacheDefault;
+  static_assert(NY > R, "every column transform has a barrier before its first LDS or twiddle read");
+  // With two LDS sets (PP), a transform whose predecessor's last loads read
+  // set 1 needs no barrier before it: its first stores go to set 0.  Tile 0
+  // keeps the barrier after the projection, whose exchange reads set 0.
+  constexpr bool kSkipPreFft = View::kBuf != 0 && FftMulti<NY, R, -1, 1, W, T, View::kBuf>::kLastLoadSet == 1;
   constexpr int kPqCP = (PP & 8) ? kCacheNt : kCacheDefault;
   __shared__ cx<T> lds_mem[View::kElems];
   __shared__ cx<T> tw_y[TwLayout<NY, R>::kAlloc];
@@ -565,8 +588,8 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     // all live across a transform.
     load_av();
     if constexpr (GF == 2) load_wv();
+    // no barrier here: the transform's first twiddle and LDS reads follow one
     stage_fft_twiddles<T, NT, NY, R>(tw_y, tw_of(pool, NY));
-    __syncthreads();
     if constexpr (GF == 2) {
       fft_multi<NY, R, -1, 2>(pq, j, lds, tw_y);
       project();
@@ -574,7 +597,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
       fft_multi<NY, R, -1, 1>(*reinterpret_cast<cx<T>(*)[1][R]>(&pq[0]), j, lds, tw_y);
       project();
       load_wv();
-      __syncthreads();
+      if (!kSkipPreFft || tile == 0) __syncthreads();
       fft_multi<NY, R, -1, 1>(*reinterpret_cast<cx<T>(*)[1][R]>(&pq[1]), j, lds, tw_y);
     }
     cx<T>* const dst = (stage == 4) ? d.wo : d.acc;
@@ -686,7 +709,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     cx<T> q3[3][R];
 #pragma unroll
     for (int f = 0; f < 3; ++f) make(q3[f], f);
-    __syncthreads();
+    if (!kSkipPreFft || stage == 0 || tile == 0) __syncthreads();
     fft_multi<NY, R, +1, 3>(q3, j, lds, tw_y);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -699,7 +722,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     for (int f = 0; f < 3; ++f) {
       cx<T> q[1][R];
       make(q[0], f);
-      __syncthreads();
+      if (!kSkipPreFft || (f == 0 && (stage == 0 || tile == 0))) __syncthreads();
       fft_multi<NY, R, +1, 1>(q, j, lds, tw_y);
       cx<T>* const dst = (f == 0) ? d.um : (f == 1) ? d.vm : d.wm;
 #pragma unroll
