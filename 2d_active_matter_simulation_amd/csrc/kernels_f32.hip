@@ -89,6 +89,7 @@ extern template hipError_t launch_row<float, 1024, 4, 8, 4, 3, 0>(const void*, i
 extern template hipError_t launch_row<float, 1024, 4, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 16, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 16, 2, 1, 4>(const void*, int, int, const void*, hipStream_t);
+extern template hipError_t launch_row<float, 1024, 4, 16, 2, 1, 5>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 16, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 1024, 4, 16, 4, 1, 0>(const void*, int, int, const void*, hipStream_t);
 extern template hipError_t launch_row<float, 2048, 2, 8, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
@@ -216,13 +217,21 @@ extern template hipError_t launch_col<float, 2048, 2, 8, 4, 2, 2, 3, 0>(const vo
 extern template hipError_t launch_col<float, 2048, 2, 8, 4, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 2, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 1>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 2>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 3>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 8>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 9>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 10>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 11>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 16>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 17>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 18>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 19>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 24>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 25>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 26>(const void*, int, int, int, const void*, hipStream_t);
+extern template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 27>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 2, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 extern template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 2>(const void*, int, int, int, const void*, hipStream_t);
@@ -328,6 +337,7 @@ const RowCfg kRow[] = {
     {1024, 4, 16, 2, 256, launch_row<T, 1024, 4, 16, 2, 1, 0>, 1, 0},
     {1024, 4, 16, 2, 256, launch_row<T, 1024, 4, 16, 2, 1, 1>, 1, 1},
     {1024, 4, 16, 2, 256, launch_row<T, 1024, 4, 16, 2, 1, 4>, 1, 4},
+    {1024, 4, 16, 2, 256, launch_row<T, 1024, 4, 16, 2, 1, 5>, 1, 5},
     {1024, 4, 16, 2, 256, launch_row<T, 1024, 4, 16, 2, 3, 0>, 3, 0},
     {1024, 4, 16, 4, 256, launch_row<T, 1024, 4, 16, 4, 1, 0>, 1, 0},
     {2048, 2, 8, 2, 512, launch_row<T, 2048, 2, 8, 2, 1, 0>, 1, 0},
@@ -457,13 +467,21 @@ const ColCfg kCol[] = {
     {2048, 2, 8, 4, 4, 512, launch_col<T, 2048, 2, 8, 4, 4, 2, 3, 0>, 2, 3, 0},
     {2048, 2, 16, 2, 2, 256, launch_col<T, 2048, 2, 16, 2, 2, 1, 1, 0>, 1, 1, 0},
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 0>, 1, 1, 0},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 1>, 1, 1, 1},
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 2>, 1, 1, 2},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 3>, 1, 1, 3},
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 8>, 1, 1, 8},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 9>, 1, 1, 9},
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 10>, 1, 1, 10},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 11>, 1, 1, 11},
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 16>, 1, 1, 16},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 17>, 1, 1, 17},
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 18>, 1, 1, 18},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 19>, 1, 1, 19},
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 24>, 1, 1, 24},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 25>, 1, 1, 25},
     {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 26>, 1, 1, 26},
+    {2048, 2, 16, 4, 2, 256, launch_col<T, 2048, 2, 16, 4, 2, 1, 1, 27>, 1, 1, 27},
     {2048, 2, 16, 4, 4, 256, launch_col<T, 2048, 2, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 0>, 1, 1, 0},
     {2048, 4, 16, 4, 4, 512, launch_col<T, 2048, 4, 16, 4, 4, 1, 1, 2>, 1, 1, 2},
@@ -776,6 +794,21 @@ const Schedule kSched[] = {
     {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 26, 0},
     {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 28, 0},
     {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 30, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 3, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 5, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 7, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 9, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 11, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 13, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 15, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 17, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 19, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 21, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 23, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 25, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 27, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 29, 0},
+    {1024, 2048, 4, 2, 16, 1, 16, 1, 1, 2, 31, 0},
     {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 2, 0},
     {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 4, 0},
     {1024, 2048, 4, 4, 8, 1, 16, 1, 1, 4, 6, 0},

@@ -18,7 +18,8 @@ template hipError_t launch_col<float, 1024, 2, 8, 2, 2, 1, 1, 0>(const void*, in
 template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 2, 3, 1>(const void*, int, int, int, const void*, hipStream_t);
 template hipError_t launch_col<float, 1024, 4, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 template hipError_t launch_col<float, 2048, 2, 8, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
-template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 10>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 3>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 19>(const void*, int, int, int, const void*, hipStream_t);
 template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 8>(const void*, int, int, int, const void*, hipStream_t);
 template hipError_t launch_col<float, 4096, 1, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 18>(const void*, int, int, int, const void*, hipStream_t);
@@ -30,5 +31,5 @@ template hipError_t launch_row<float, 512, 2, 8, 4, 3, 0>(const void*, int, int,
 template hipError_t launch_row<float, 512, 4, 8, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
 template hipError_t launch_row<float, 512, 4, 16, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 template hipError_t launch_row<float, 1024, 4, 8, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
-template hipError_t launch_row<float, 1024, 4, 16, 4, 1, 0>(const void*, int, int, const void*, hipStream_t);
+template hipError_t launch_row<float, 1024, 4, 16, 2, 3, 0>(const void*, int, int, const void*, hipStream_t);
 }  // namespace af

@@ -18,6 +18,7 @@ template hipError_t launch_col<float, 1024, 2, 8, 4, 2, 1, 1, 8>(const void*, in
 template hipError_t launch_col<float, 1024, 4, 8, 2, 4, 2, 3, 0>(const void*, int, int, int, const void*, hipStream_t);
 template hipError_t launch_col<float, 2048, 1, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
+template hipError_t launch_col<float, 2048, 2, 16, 4, 2, 1, 1, 16>(const void*, int, int, int, const void*, hipStream_t);
 template hipError_t launch_col<float, 2048, 2, 16, 4, 4, 1, 1, 0>(const void*, int, int, int, const void*, hipStream_t);
 template hipError_t launch_col<float, 2048, 4, 16, 4, 4, 1, 1, 26>(const void*, int, int, int, const void*, hipStream_t);
 template hipError_t launch_col<float, 4096, 2, 16, 4, 4, 1, 1, 8>(const void*, int, int, int, const void*, hipStream_t);
@@ -34,5 +35,5 @@ template hipError_t launch_row<float, 512, 4, 4, 2, 3, 1>(const void*, int, int,
 template hipError_t launch_row<float, 512, 4, 8, 4, 3, 0>(const void*, int, int, const void*, hipStream_t);
 template hipError_t launch_row<float, 1024, 2, 16, 2, 1, 0>(const void*, int, int, const void*, hipStream_t);
 template hipError_t launch_row<float, 1024, 4, 16, 2, 1, 1>(const void*, int, int, const void*, hipStream_t);
-template hipError_t launch_row<float, 2048, 4, 16, 4, 1, 4>(const void*, int, int, const void*, hipStream_t);
+template hipError_t launch_row<float, 2048, 4, 16, 4, 1, 0>(const void*, int, int, const void*, hipStream_t);
 }  // namespace af
