@@ -53,10 +53,12 @@ COL_FIELDS = (7 + 8 + 8 + 7) / 4.0
 
 def col_fields(af, n, prec):
     """Column-pass transfers per launch, plus the integrating-factor table an
-    FP32 run reads when its column radix is <= 8 (4 B per mode per plane:
-    2 planes at stages 1 and 3, 1 at stage 2, in units of one FP32 field)."""
+    FP32 run reads when its column radix is <= 8 or its schedule sets option
+    16 (4 B per mode per plane: 2 planes at stages 1 and 3, 1 at stage 2, in
+    units of one FP32 field)."""
     extra = 0.0
-    if prec == 32 and af.active_schedule(prec, n)[4] <= 8:
+    sc = af.active_schedule(prec, n)
+    if prec == 32 and (sc[4] <= 8 or sc[8] & 16):
         extra = (2 + 1 + 2 + 0) / 4.0 * 0.5
     return COL_FIELDS + extra
 
