@@ -441,13 +441,13 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   constexpr int GMAX = GF > GI ? GF : GI;
   static_assert((GF == 1 || GF == 2) && (GI == 1 || GI == 3), "field groups");
   using View = LdsView<T, NY, W, R, GMAX, PP & 1>;
-  constexpr int kCP = (PP & 2) ? kCacheNt : kCacheDefault;
+  constexpr int kCP = (PP & 2) ? kCacheNt : kCacheDefault;     // field stores
+  constexpr int kPqCP = (PP & 8) ? kCacheNt : kCacheDefault;   // P, Q loads
   static_assert(NY > R, "every column transform has a barrier before its first LDS or twiddle read");
   // With two LDS sets (PP), a transform whose predecessor's last loads read
   // set 1 needs no barrier before it: its first stores go to set 0.  Tile 0
   // keeps the barrier after the projection, whose exchange reads set 0.
   constexpr bool kSkipPreFft = View::kBuf != 0 && FftMulti<NY, R, -1, 1, W, T, View::kBuf>::kLastLoadSet == 1;
-  constexpr int kPqCP = (PP & 8) ? kCacheNt : kCacheDefault;
   __shared__ cx<T> lds_mem[View::kElems];
   __shared__ cx<T> tw_y[TwLayout<NY, R>::kAlloc];
   __shared__ double mean_in[4];   // tile 0: M, M_old
