@@ -61,6 +61,17 @@ def time_schedule(prec, n, idx, steps, instances, check_steps=4):
     return dt / steps * 1e3, state
 
 
+def merge_table(lines, existing):
+    """Table text from the header + tuned `lines`, keeping the lines of the
+    existing table text for (precision, nx, ny) not tuned now."""
+    tuned = {tuple(ln.split()[:3]) for ln in lines[1:]}
+    out = list(lines)
+    for ln in existing.splitlines():
+        if ln.strip() and not ln.startswith("#") and tuple(ln.split()[:3]) not in tuned:
+            out.append(ln)
+    return "\n".join(out) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--sizes", type=int, nargs="+", default=[256, 512, 1024, 2048, 4096])
@@ -95,14 +106,7 @@ def main():
                 note = f" ({args.instances} instances)" if args.instances > 1 else ""
                 lines.append(f"{prec} {n} {n} " + " ".join(map(str, best[1])) + f"   # {best[0]:.4f}{note}")
                 print(f"  -> best {best[1]} {best[0]:.4f} ms/step", flush=True)
-    # keep the existing table's lines for the (precision, size) pairs not tuned now
-    tuned = {tuple(ln.split()[:3]) for ln in lines[1:]}
-    if os.path.exists(args.out):
-        for ln in open(args.out):
-            key = tuple(ln.split()[:3])
-            if ln.strip() and not ln.startswith("#") and key not in tuned:
-                lines.append(ln.rstrip("\n"))
-    text = "\n".join(lines) + "\n"
+    text = merge_table(lines, open(args.out).read() if os.path.exists(args.out) else "")
     if args.dry:
         print(text)
     else:

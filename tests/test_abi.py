@@ -94,3 +94,16 @@ def test_schedule_struct_fields(af):
             assert bx % cw == 0 and rr in (4, 8, 16) and cr in (4, 8, 16)
             assert 0 <= opts < 32 and (prec == 32 or opts in (0, 1))
     assert any(sc[8] & 6 for sc in af.schedules(32, 1024))
+
+
+def test_autotune_merge_keeps_untuned_lines():
+    """Re-tuning some sizes rewrites only their lines of the table."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "autotune", os.path.join(ROOT, "2d_active_matter_simulation_amd", "autotune.py"))
+    at = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(at)
+    old = "# header\n32 1024 1024 4 2 4 3 4 2 3 2 0   # 0.1\n64 1024 1024 4 2 4 3 4 2 3 2 0   # 0.2\n"
+    new = at.merge_table(["# header", "32 1024 1024 4 2 4 3 4 2 1 2 15   # 0.09"], old)
+    rows = [ln for ln in new.splitlines() if not ln.startswith("#")]
+    assert rows == ["32 1024 1024 4 2 4 3 4 2 1 2 15   # 0.09", "64 1024 1024 4 2 4 3 4 2 3 2 0   # 0.2"]
