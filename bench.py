@@ -247,6 +247,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the served-workload leg")
     ap.add_argument("--grid", type=int, default=1024,
                     help="grid of the single-run leg (default 1024, the headline; other sizes for profiling)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="profiling aid: only step --batch FP32 instances of --grid in the ensemble engine "
+                         "(--steps steps) and print their throughput")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device-to-device copies of 256 MiB (the reference kernel for scaling "
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes), then exit")
@@ -269,6 +272,9 @@ def main():
     af.lib()
 
     N = args.grid
+    if args.batch > 0:
+        print(json.dumps({"batched": batched_leg(af, N, args.batch, args.steps)}))
+        return
     head = run_workload(af, N, 32, args.steps, args.warmup)
     name, alg, kms, achieved = roofline(af, head)
     traffic = read_traffic(f"{N}_fp32")
