@@ -194,3 +194,28 @@ def test_parity_fp64_1024_two_steps(af, orc):
     p = af.params(N, dt=1e-4, precision=64, **JW)
     w0 = af.ic_bandlimited(N, seed=1, amp=2.0, kmax=12.0)
     compare(af, orc, p, w0, (0.0, 0.0), (2,), 1e-12, 1e-12, nthreads=8)
+
+
+@pytest.mark.parametrize("N", [2048, 4096])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_parity_large_one_step(af, orc, prec, N):
+    """The tuned 2048^2 and 4096^2 schedules (radix 8/16 rows and columns,
+    cache and table options) against the oracle directly: FP64 to 1e-12, FP32
+    to its rounding (the FP64 oracle as reference)."""
+    p = af.params(N, dt=5e-5, precision=prec, **JW)
+    q = af.AfParams.from_buffer_copy(p)
+    q.precision = 64
+    w0 = af.ic_bandlimited(N, seed=3, amp=2.0, kmax=12.0)
+    run = af.Run(p)
+    run.set_state(w0, (0.05 * N * N, -0.02 * N * N))
+    e = run.step(1, energy=True)
+    w, m = run.get_state()
+    run.close()
+    o = orc.Oracle.from_params(q, nthreads=16)
+    o.set_state(w0, (0.05 * N * N, -0.02 * N * N))
+    eo = o.step(1, energy=True)
+    wo, mo = o.get_state()
+    tol = 1e-12 if prec == 64 else 2e-6
+    assert rel_l2(w, wo) <= tol, rel_l2(w, wo)
+    assert abs(e[0] - eo[0]) <= tol * abs(eo[0])
+    assert np.allclose(m, mo, rtol=10 * tol)
