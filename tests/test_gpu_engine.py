@@ -133,3 +133,28 @@ def test_engine_schedule_matches_single_run_to_rounding(af):
     run.close()
     assert rel_l2(we, wr) <= 1e-5
     assert np.allclose(me, mr, rtol=1e-5, atol=1e-6 * n * n)
+
+
+def test_engine_batched_schedule_2048_against_oracle(af, orc):
+    """Two 2048^2 FP32 runs in one engine (the tuned batched schedule: 4x4
+    bricks, radix-16 columns, 4-column blocks, streaming options) against the
+    FP64 oracle after one step, each to FP32 rounding."""
+    N = 2048
+    kw = dict(lam=3.5, alpha=-0.8, beta=0.01, gamma0=-0.045, gamma2=2.5e-5)
+    p = af.params(N, dt=5e-5, precision=32, **kw)
+    q = af.AfParams.from_buffer_copy(p)
+    q.precision = 64
+    eng = af.Engine(1 << 34)
+    ws = [af.ic_bandlimited(N, seed=s, amp=2.0, kmax=12.0) for s in (5, 6)]
+    slots = [eng.add(p, w, (0.05 * N * N, 0.0)) for w in ws]
+    eng.step(1)
+    for w0, s in zip(ws, slots):
+        w, m = eng.get_state(s)
+        o = orc.Oracle.from_params(q, nthreads=16)
+        o.set_state(w0, (0.05 * N * N, 0.0))
+        o.step(1)
+        wo, mo = o.get_state()
+        err = np.linalg.norm(w - wo) / np.linalg.norm(wo)
+        assert err <= 2e-6, err
+        assert np.allclose(m, mo, rtol=2e-5)
+    eng.close()
