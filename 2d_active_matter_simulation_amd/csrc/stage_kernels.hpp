@@ -180,14 +180,46 @@ __device__ __forceinline__ int xcd_tile_order(int i, int n) {
   }
 }
 
+// Wave sum of NV doubles, valid in lane 0 of each (possibly partial) wave,
+// in a fixed order.  DPP row permutations fold the 16 lanes of each row
+// (neighbour, pair of neighbours, half-row mirror, row mirror: every lane
+// ends with its row's sum), then lane 0 adds the sums of rows 1-3 read with
+// readlane.  A VALU move per 32-bit half and step instead of an LDS-unit
+// permute keeps the chain short.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+template <int NV, int NT>
+__device__ __forceinline__ void wave_sum_lane0(double (&v)[NV]) {
+  constexpr int WW = NT < 64 ? NT : 64;   // lanes of a (possibly partial) wave
+  static_assert((WW & (WW - 1)) == 0, "power-of-two lane count");
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    if constexpr (WW >= 2) v[k] += dpp_f64<0xB1>(v[k]);    // quad_perm [1,0,3,2]
+    if constexpr (WW >= 4) v[k] += dpp_f64<0x4E>(v[k]);    // quad_perm [2,3,0,1]
+    if constexpr (WW >= 8) v[k] += dpp_f64<0x141>(v[k]);   // row_half_mirror
+    if constexpr (WW >= 16) v[k] += dpp_f64<0x140>(v[k]);  // row_mirror
+  }
+  // every lane adds, so the row sums are read from the values before the adds
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double rowsum = v[k];
+#pragma unroll
+    for (int row = 1; row < WW / 16; ++row) v[k] += readlane_f64(rowsum, 16 * row);
+  }
+}
+
 // Deterministic block sum of NV doubles; result valid in thread 0.
 template <int NV, int NT>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* red /* LDS, NV * waves */) {
-  constexpr int WW = NT < 64 ? NT : 64;   // lanes of a (possibly partial) wave
-#pragma unroll
-  for (int k = 0; k < NV; ++k)
-#pragma unroll
-    for (int off = WW / 2; off >= 1; off >>= 1) v[k] += __shfl_xor(v[k], off, WW);
+  wave_sum_lane0<NV, NT>(v);
   constexpr int NW = (NT + 63) / 64;
   if constexpr (NW > 1) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -206,16 +238,12 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red /* LDS, N
 }
 
 // First half of a deterministic block sum that needs no barrier of its own:
-// each wave folds its lanes (butterfly) and lane 0 parks the wave sums in
+// each wave sums its lanes and lane 0 parks the wave sums in
 // red[wave * NV + k]; after a later barrier, lds_wave_sum adds them in wave
 // order.
 template <int NV, int NT>
 __device__ __forceinline__ void wave_fold_to_lds(double (&v)[NV], double* red /* LDS, NV * waves */) {
-  constexpr int WW = NT < 64 ? NT : 64;   // lanes of a (possibly partial) wave
-#pragma unroll
-  for (int k = 0; k < NV; ++k)
-#pragma unroll
-    for (int off = WW / 2; off >= 1; off >>= 1) v[k] += __shfl_xor(v[k], off, WW);
+  wave_sum_lane0<NV, NT>(v);
   if ((threadIdx.x & 63) == 0)
 #pragma unroll
     for (int k = 0; k < NV; ++k) red[(threadIdx.x >> 6) * NV + k] = v[k];
