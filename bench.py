@@ -80,7 +80,7 @@ def read_traffic(workload_tag):
     return vals or None
 
 
-def run_workload(af, n, prec, steps, warmup, seed=1):
+def run_workload(af, n, prec, steps, warmup, seed=1, host_io=False):
     p = af.params(n, dt=1e-4 if n <= 1024 else 5e-5, precision=prec, **PHYS)
     run = af.Run(p)
     run.set_state(af.ic_bandlimited(n, seed=seed, amp=1e-3, kmax=10.0))
@@ -90,13 +90,27 @@ def run_workload(af, n, prec, steps, warmup, seed=1):
     e = run.step(steps, energy=True)          # returns after the device finished
     wall = time.perf_counter() - t0
     assert np.all(np.isfinite(e))
+    io = None
+    if host_io:
+        # the C ABI hands the state over in host memory (FP64 half spectrum):
+        # time a download and an upload of the current state (median of 5)
+        gets, sets = [], []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            w, m = run.get_state()
+            gets.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            run.set_state(w, m)
+            sets.append(time.perf_counter() - t0)
+        io = dict(get_state_ms=float(np.median(gets)) * 1e3, set_state_ms=float(np.median(sets)) * 1e3,
+                  state_bytes=int(w.nbytes + m.nbytes))
     # per-kernel durations: HIP events around every launch on the run's stream
     kprof = max(10, min(steps, 200))
     row_ms, col_ms, tot_ms = run.profile(kprof)
     run.close()
     return dict(n=n, prec=prec, steps=steps, wall=wall, ms_per_step=wall / steps * 1e3,
                 value=n * n * steps / wall, row_ms=row_ms, col_ms=col_ms, prof_steps=kprof,
-                prof_ms_per_step=tot_ms / kprof)
+                prof_ms_per_step=tot_ms / kprof, host_io=io)
 
 
 def roofline(af, r):
@@ -275,7 +289,7 @@ def main():
     if args.batch > 0:
         print(json.dumps({"batched": batched_leg(af, N, args.batch, args.steps)}))
         return
-    head = run_workload(af, N, 32, args.steps, args.warmup)
+    head = run_workload(af, N, 32, args.steps, args.warmup, host_io=True)
     name, alg, kms, achieved = roofline(af, head)
     traffic = read_traffic(f"{N}_fp32")
     tr_bytes = None
@@ -319,6 +333,15 @@ def main():
                      "step_model_frac": step_model_bytes(N) / (head["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "extra": extra,
     }
+    io = head["host_io"]
+    if io:
+        # PCIe-inclusive rates (never `value`): one upload + one download around
+        # the timed steps, and a download every 500 steps (the served snapshot cadence)
+        step_s = head["ms_per_step"] * 1e-3
+        io["with_one_upload_and_download"] = N * N * args.steps / (
+            args.steps * step_s + (io["get_state_ms"] + io["set_state_ms"]) * 1e-3)
+        io["with_download_every_500_steps"] = N * N * 500 / (500 * step_s + io["get_state_ms"] * 1e-3)
+        out["host_io"] = io
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(af)
         out["batched"] = {}
