@@ -436,5 +436,32 @@ def snapshot_read(path, nx, ny):
     return w, m
 
 
+def snapshot_image(path, nx, ny, spectral=False):
+    """The array the reference's plotting script draws for one snapshot
+    (image.py:49-69), without the plotting.
+
+    spectral=False: the real-space vorticity, the inverse real DFT of the
+    stored half spectrum normalised by 1/(Nx Ny) (image.py:67; the stored
+    values are unnormalised DFT coefficients, so this is the physical field).
+
+    spectral=True: |w^| on the full (kx, ky) plane, Ny x Nx.  Column c holds
+    kx = c - s and row r the half-spectrum row (r - s) mod Ny, with
+    s = Nx/2 - 1 for both axes (image.py:64 shifts the rows by Nx//2 - 1 also
+    when Ny != Nx; kept for identical images); a column with kx < 0 shows the
+    Hermitian partner |w^(-kx, -ky)| (image.py:55-62)."""
+    w, _ = snapshot_read(path, nx, ny)
+    if not spectral:
+        return np.fft.irfft2(w, s=(ny, nx))
+    shift = nx // 2 - 1
+    kx = np.arange(nx) - shift                       # -(Nx/2 - 1) .. Nx/2
+    row = (np.arange(ny) - shift) % ny               # half-spectrum row of each image row
+    mag = np.abs(w)
+    img = np.empty((ny, nx))
+    pos = kx >= 0
+    img[:, pos] = mag[row][:, kx[pos]]
+    img[:, ~pos] = mag[(-row) % ny][:, -kx[~pos]]
+    return img
+
+
 def energy_append(path, t, E):
     _check(lib().af_energy_append(str(path).encode(), t, E))

@@ -91,6 +91,44 @@ def test_snapshot_default_precision_is_ostream(af, tmp_path):
     assert np.allclose(w2, [[1.23457e6 + 0.000123457j, -0.5, 3 + 1e-20j]])
 
 
+def test_snapshot_image_real_field(af, tmp_path):
+    """Real-space image (image.py:67): stored values are unnormalised DFT
+    coefficients, so a single stored coefficient Nx*Ny/2 at kx = 1 is cos(x)
+    on the grid, and any real field survives rfft2 -> snapshot -> image."""
+    ny, nx = 8, 16
+    w = np.zeros((ny, nx // 2 + 1), np.complex128)
+    w[0, 1] = nx * ny / 2
+    path = tmp_path / "cos.bin"
+    af.snapshot_write(path, w, (0.0, 0.0), precision=17)
+    img = af.snapshot_image(path, nx, ny)
+    x = 2 * np.pi * np.arange(nx) / nx
+    assert img.shape == (ny, nx)
+    assert np.allclose(img, np.cos(x)[None, :], atol=1e-14)
+    f = np.random.default_rng(1).standard_normal((ny, nx))
+    af.snapshot_write(path, np.fft.rfft2(f), (0.0, 0.0), precision=17)
+    assert np.allclose(af.snapshot_image(path, nx, ny), f, atol=1e-13)
+
+
+@pytest.mark.parametrize("y0,x0", [(3, 5), (0, 0), (6, 8), (5, 0), (0, 7)])
+def test_snapshot_image_spectral_placement(af, tmp_path, y0, x0):
+    """Spectral image (image.py:53-65): kx = -(Nx/2-1) .. Nx/2 left to right,
+    rows shifted by Nx/2 - 1 (also on this Ny != Nx grid), so one stored mode
+    (kx0, ky0) lights (row (ky0 + s) mod Ny, col kx0 + s) and, when
+    0 < kx0 < Nx/2, its Hermitian partner at (-ky0, -kx0)."""
+    ny, nx = 8, 16
+    s = nx // 2 - 1
+    w = np.zeros((ny, nx // 2 + 1), np.complex128)
+    w[y0, x0] = 3.0 - 4.0j
+    path = tmp_path / "mode.bin"
+    af.snapshot_write(path, w, (0.0, 0.0), precision=17)
+    img = af.snapshot_image(path, nx, ny, spectral=True)
+    want = np.zeros((ny, nx))
+    want[(y0 + s) % ny, x0 + s] = 5.0
+    if 0 < x0 < nx // 2:
+        want[(-y0 + s) % ny, -x0 + s] = 5.0
+    assert np.array_equal(img, want)
+
+
 def test_energy_line_format(af, tmp_path):
     path = tmp_path / "energy.txt"
     af.energy_append(path, 0.002, 1.0 / 3.0)
