@@ -201,12 +201,27 @@ af_status af_serve(const af_request* reqs, int n, size_t hbm_budget, int max_tic
     std::vector<int> slots;
     for (const Resident& r : res) slots.push_back(r.slot);
     st = af_engine_step_slots(eng, slots.data(), (int)slots.size(), (int)k);
+    // Every listed run has taken its k steps even when the watchdog reports
+    // NaN.  A run whose state went NaN fails alone, as the reference stops
+    // that run (.cu:850-861): status AF_ENAN, no finish time; the others go on.
+    std::vector<char> bad(res.size(), 0);
+    if (st == AF_ENAN) {
+      st = AF_OK;
+      for (size_t i = 0; i < res.size() && st == AF_OK; ++i) {
+        const Slot& sl = eng->slots[res[i].slot];
+        MeanState ms;
+        hipError_t e = hipMemcpy(&ms, sl.base + sl.g.off_mean, sizeof(ms), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = hip_fail(e, "mean download");
+        else bad[i] = ms.nan_step >= 0;
+      }
+    }
     if (st != AF_OK) break;
     ++ticks;
     tnow = now();
     // 3. latencies, snapshots, completions
     std::vector<Resident> keep;
-    for (Resident& r : res) {
+    for (size_t i = 0; i < res.size(); ++i) {
+      Resident& r = res[i];
       const af_request& rq = reqs[r.req];
       r.steps += k;
       const double per = (tnow - r.last) / (double)k;
@@ -214,6 +229,12 @@ af_status af_serve(const af_request* reqs, int n, size_t hbm_budget, int max_tic
       lat_all.push_back({per, k});
       r.last = tnow;
       gps += (double)rq.params.nx * rq.params.ny * (double)k;
+      if (bad[i]) {
+        results[r.req].status = AF_ENAN;
+        (void)deliver(true);   // its earlier snapshots precede its removal
+        if ((st = af_engine_remove(eng, r.slot)) != AF_OK) break;
+        continue;
+      }
       if (rq.snapshot_every > 0 && r.steps % rq.snapshot_every == 0)
         if ((st = start_snapshot(r.req, r)) != AF_OK) break;
       if (r.steps >= rq.horizon) {

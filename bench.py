@@ -201,7 +201,9 @@ def serving_leg(af, rate, window=0.5, seed=11, horizon_scale=0.1):
     nsnap = [0]
     res, st, _ = af.serve(reqs, hbm_budget=200 << 30, max_tick_steps=50,
                           on_snapshot=lambda *a: nsnap.__setitem__(0, nsnap[0] + 1))
+    failed = sum(1 for r in res if r["status"] != 0)   # e.g. a run whose state went NaN
     return {"rate_per_s": rate, "window_s": window, "requests": len(reqs), "completed": st["completed"],
+            "failed": failed,
             "horizon": f"U{{1k..50k}} x {horizon_scale}", "throughput": st["throughput"],
             "step_p50_ms": st["step_p50_s"] * 1e3, "step_p99_ms": st["step_p99_s"] * 1e3,
             "ttc_p50_s": st["ttc_p50_s"], "ttc_p99_s": st["ttc_p99_s"], "queue_p50_s": st["queue_p50_s"],
@@ -267,6 +269,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device-to-device copies of 256 MiB (the reference kernel for scaling "
                          "rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes), then exit")
+    ap.add_argument("--serving-requests", type=int, default=0,
+                    help="only run the served workload at 10, 50 and 200 requests/s with the full "
+                         "U{1k..50k} horizons, each arrival window sized for about this many requests, "
+                         "printing one JSON line per rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -286,6 +292,11 @@ def main():
     af.lib()
 
     N = args.grid
+    if args.serving_requests > 0:
+        for r in (10, 50, 200):
+            print(json.dumps({"serving": serving_leg(af, r, window=args.serving_requests / r, horizon_scale=1.0)}),
+                  flush=True)
+        return
     if args.batch > 0:
         print(json.dumps({"batched": batched_leg(af, N, args.batch, args.steps)}))
         return

@@ -194,7 +194,9 @@ typedef void (*af_snapshot_fn)(void* user, int request, long long step, const do
  * copied back asynchronously, finished runs retired so waiting requests can
  * enter.  final_states (optional) holds per request NULL or a buffer of
  * 2*Ny*(Nx/2+1) doubles for the final state.  A request's result is bitwise
- * the result of running it alone. */
+ * the result of running it alone.  A request whose state goes NaN fails alone
+ * (result status AF_ENAN, finish_s -1, the check_func stop of .cu:850-861);
+ * the others go on and af_serve returns AF_OK. */
 af_status af_serve(const af_request* reqs, int n, size_t hbm_budget, int max_tick_steps, af_snapshot_fn fn,
                    void* user, double** final_states, af_request_result* results, af_serve_stats* stats);
 

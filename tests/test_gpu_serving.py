@@ -58,3 +58,21 @@ def test_serve_rejects_request_larger_than_budget(af):
     reqs = [af.Request(arrival_s=0.0, params=p, horizon=5, seed=1)]
     res, st, _ = af.serve(reqs, hbm_budget=af.device_bytes(p) // 2, max_tick_steps=4)
     assert res[0]["status"] == af.AF_ENOMEM and st["completed"] == 0
+
+
+def test_serve_nan_request_fails_alone(af):
+    """A request whose state blows up (the parameters of test_nan_watchdog)
+    ends with AF_ENAN and no finish time; the other requests complete with
+    the same results as in a mix without it."""
+    reqs = _reqs(af)[:6]
+    bad = af.Request(arrival_s=reqs[1].arrival_s,
+                     params=af.params(64, dt=0.5, lam=50.0, alpha=-50.0, beta=-5.0, precision=32),
+                     horizon=200, snapshot_every=0, seed=1, ic_amp=100.0, ic_kmax=8.0)
+    res, st, finals = af.serve(reqs + [bad], hbm_budget=1 << 33, max_tick_steps=16, keep_final=True)
+    assert res[-1]["status"] == af.AF_ENAN and res[-1]["finish_s"] < 0 <= res[-1]["admit_s"]
+    assert st["completed"] == len(reqs)
+    ref, _, ref_finals = af.serve(reqs, hbm_budget=1 << 33, max_tick_steps=16, keep_final=True)
+    for i in range(len(reqs)):
+        assert res[i]["status"] == 0
+        assert np.array_equal(finals[i], ref_finals[i]), i
+        assert res[i]["M"] == ref[i]["M"]
