@@ -123,34 +123,57 @@ def roofline(af, r):
     return name, alg, ms, achieved
 
 
-def ensemble_leg(af, steps=200, seed=7):
+def ensemble_leg(af, steps=200, seed=7, measured_sequential=False):
     """Config 4: 64 instances (32 x 256^2, 24 x 512^2, 8 x 1024^2), seeded random
     alpha, beta, lambda0, FP32, stepped together in shared launches; compared
     with the same instances run one after another through the single-run path
-    (timed per size class and summed)."""
+    (timed per size class and summed, or with measured_sequential every one
+    of the 64 runs stepped alone and timed, and its final state compared)."""
     rng = np.random.default_rng(seed)
     mix = [(256, 32), (512, 24), (1024, 8)]
     eng = af.Engine(200 << 30)
     pts = 0
+    runs = []
     for n, cnt in mix:
         for i in range(cnt):
             p = af.params(n, dt=2e-4 if n <= 256 else 1e-4, lam=float(rng.uniform(1, 5)),
                           alpha=float(rng.uniform(-1.2, -0.4)), beta=float(rng.uniform(0.005, 0.02)),
                           gamma0=-0.045, gamma2=2.5e-5, precision=32)
-            eng.add(p, af.ic_bandlimited(n, seed=int(rng.integers(1, 1 << 30)), amp=1e-3, kmax=10.0))
+            ic = af.ic_bandlimited(n, seed=int(rng.integers(1, 1 << 30)), amp=1e-3, kmax=10.0)
+            runs.append((eng.add(p, ic), p, ic))
             pts += n * n
     eng.step(10)
     t0 = time.perf_counter()
     eng.step(steps)
     wall = time.perf_counter() - t0
     value = pts * steps / wall
+    out = {}
+    if measured_sequential:
+        seq, worst = 0.0, 0.0
+        for slot, p, ic in runs:
+            run = af.Run(p)
+            run.set_state(ic)
+            run.step(10)
+            t0 = time.perf_counter()
+            run.step(steps)
+            seq += time.perf_counter() - t0
+            w, _ = run.get_state()
+            run.close()
+            we, _ = eng.get_state(slot)
+            worst = max(worst, float(np.linalg.norm(we - w) / np.linalg.norm(w)))
+        # the engine's batched schedules differ from the single-run ones, so the
+        # two agree to FP32 rounding grown over the steps, not bitwise
+        out["max_rel_diff_engine_vs_alone"] = worst
     eng.close()
-    seq = 0.0
-    for n, cnt in mix:
-        r = run_workload(af, n, 32, max(50, steps // 2), 10)
-        seq += cnt * (r["ms_per_step"] * 1e-3) * steps
+    if not measured_sequential:
+        seq = 0.0
+        for n, cnt in mix:
+            r = run_workload(af, n, 32, max(50, steps // 2), 10)
+            seq += cnt * (r["ms_per_step"] * 1e-3) * steps
     fb = sum(cnt * field_bytes(n, 32) for n, cnt in mix)
     return {"instances": 64, "mix": "32x256^2 + 24x512^2 + 8x1024^2 FP32", "steps": steps,
+            "sequential": "measured, each run alone" if measured_sequential else "per-size timings x counts",
+            **out,
             "value": value, "ms_per_step": wall / steps * 1e3, "sequential_value": pts * steps / seq,
             "speedup_vs_sequential": seq / wall,
             "step_bytes_50_transfers": 50 * fb,
@@ -273,6 +296,9 @@ def main():
                     help="only run the served workload at 10, 50 and 200 requests/s with the full "
                          "U{1k..50k} horizons, each arrival window sized for about this many requests, "
                          "printing one JSON line per rate")
+    ap.add_argument("--ensemble-full", type=int, default=0,
+                    help="only run the 64-instance ensemble for this many steps and compare it with each "
+                         "of the 64 runs stepped alone (measured), printing one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -292,6 +318,9 @@ def main():
     af.lib()
 
     N = args.grid
+    if args.ensemble_full > 0:
+        print(json.dumps({"ensemble": ensemble_leg(af, steps=args.ensemble_full, measured_sequential=True)}))
+        return
     if args.serving_requests > 0:
         for r in (10, 50, 200):
             print(json.dumps({"serving": serving_leg(af, r, window=args.serving_requests / r, horizon_scale=1.0)}),
