@@ -182,6 +182,28 @@ def ensemble_leg(af, steps=200, seed=7, measured_sequential=False):
             / HBM_PEAK_GBS}
 
 
+def config3_leg(af, steps=5000):
+    """Config 3 at its full length: single FP32 runs of 2048^2 and 4096^2 for
+    lambda0 in {2, 3.5, 5} (alpha -0.8, beta 0.01, Gamma0 -0.045, Gamma2 2.5e-5),
+    dt 5e-5, `steps` steps with per-step energy from the seeded band-limited
+    IC; one line per run."""
+    out = []
+    for n in (2048, 4096):
+        for lam in (2.0, 3.5, 5.0):
+            p = af.params(n, dt=5e-5, lam=lam, alpha=-0.8, beta=0.01, gamma0=-0.045, gamma2=2.5e-5, precision=32)
+            run = af.Run(p)
+            run.set_state(af.ic_bandlimited(n, seed=1, amp=1e-3, kmax=10.0))
+            t0 = time.perf_counter()
+            e = run.step(steps, energy=True)
+            wall = time.perf_counter() - t0
+            run.close()
+            out.append({"grid": [n, n], "lambda0": lam, "steps": steps, "value": n * n * steps / wall,
+                        "ms_per_step": wall / steps * 1e3, "energy_first": float(e[0]), "energy_last": float(e[-1]),
+                        "finite": bool(np.all(np.isfinite(e)))})
+            print(json.dumps({"config3": out[-1]}), flush=True)
+    return out
+
+
 def batched_leg(af, n, count, steps, seed=3):
     """Config 3 sizes stepped as a batch: `count` FP32 instances of n x n with
     lambda0 cycling over {2, 3.5, 5} (other parameters as the headline), in
@@ -299,6 +321,9 @@ def main():
     ap.add_argument("--ensemble-full", type=int, default=0,
                     help="only run the 64-instance ensemble for this many steps and compare it with each "
                          "of the 64 runs stepped alone (measured), printing one JSON line")
+    ap.add_argument("--config3", type=int, default=0,
+                    help="only run single FP32 runs of 2048^2 and 4096^2 for lambda0 in {2, 3.5, 5} for this "
+                         "many steps (from the first step, energy on), one JSON line per run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -318,6 +343,9 @@ def main():
     af.lib()
 
     N = args.grid
+    if args.config3 > 0:
+        config3_leg(af, args.config3)
+        return
     if args.ensemble_full > 0:
         print(json.dumps({"ensemble": ensemble_leg(af, steps=args.ensemble_full, measured_sequential=True)}))
         return
