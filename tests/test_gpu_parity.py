@@ -53,6 +53,16 @@ def test_parity_fp64_config1(af, orc, seed):
     compare(af, orc, p, w0, (0.0, 0.0), (1, 9, 90), 1e-12, 1e-12)
 
 
+def test_parity_fp64_config1_energy_series(af, orc):
+    """256^2 FP64 config in full: the per-step energy series of 2000 steps and
+    the final state match the oracle (the flow is in its linear growth phase,
+    where rounding differences do not grow relative to the field)."""
+    p = af.params(256, dt=2e-4, precision=64, **JW)
+    w0 = af.ic_bandlimited(256, seed=1, amp=1e-3, kmax=10.0)
+    worst = compare(af, orc, p, w0, (0.0, 0.0), (2000,), 1e-11, 1e-11, nthreads=8)
+    print(f"\n256^2 FP64, 2000 steps: rel L2 vs oracle {worst:.2e}")
+
+
 def test_parity_fp64_developed(af, orc):
     """O(1) amplitude so the nonlinear and cubic terms are large, with a mean flow."""
     N = 128
