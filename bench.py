@@ -256,6 +256,37 @@ def serving_leg(af, rate, window=0.5, seed=11, horizon_scale=0.1):
             "wall_s": st["wall_s"]}
 
 
+def cpu_configs():
+    """The oracle per workload of the configuration list, on all host cores
+    (up to 16) and on one thread: median time of single steps (energy on)
+    after warm-up steps (10 on all cores, 2 on one thread), one JSON line per
+    leg.  Steps on one thread are fewer to bound the run."""
+    from oracle.oracle_py import Oracle
+    af = importlib.import_module("2d_active_matter_simulation_amd")
+    threads = min(os.cpu_count() or 1, 16)
+    legs = [("config 1: 256^2 FP64, dt 2e-4, all 2000 steps", 256, 64, 2e-4, 2000, 200),
+            ("config 2: 1024^2 FP64, dt 1e-4, first 200 steps", 1024, 64, 1e-4, 200, 5),
+            ("config 2: 1024^2 FP32, dt 1e-4, first 200 steps", 1024, 32, 1e-4, 200, 5),
+            ("2048^2 FP32, dt 5e-5, 10 steps", 2048, 32, 5e-5, 10, 2)]
+    out = []
+    for name, n, prec, dt, steps, steps1 in legs:
+        for nt, k, warm in ((threads, steps, 10), (1, steps1, 2)):
+            o = Oracle(n, dt=dt, lam=PHYS["lam"], beta=PHYS["beta"], c0=-PHYS["alpha"], c2=-PHYS["gamma0"],
+                       c4=-PHYS["gamma2"], precision=prec, nthreads=nt)
+            o.set_state(af.ic_bandlimited(n, seed=1, amp=1e-3, kmax=10.0))
+            o.step(warm)
+            times = []
+            for _ in range(k):
+                t0 = time.perf_counter()
+                o.step(1, energy=True)
+                times.append(time.perf_counter() - t0)
+            med = float(np.median(times))
+            out.append({"workload": name, "threads": nt, "steps_timed": k, "warmup": warm,
+                        "median_ms_per_step": med * 1e3, "value": n * n / med, "unit": "grid-pt-steps/s"})
+            print(json.dumps({"cpu_oracle": out[-1]}), flush=True)
+    return out
+
+
 def cpu_baseline(n=1024, prec=32, target_s=10.0):
     """The oracle (C++ restatement of the reference step) on the host cores,
     timed over enough steps for ~target_s of CPU work (at least 3)."""
@@ -287,14 +318,16 @@ def cpu_baseline(n=1024, prec=32, target_s=10.0):
     o1.step(steps1, energy=True)
     wall1 = time.perf_counter() - t0
     out["single_thread"] = {"value": n * n * steps1 / wall1, "cores": 1, "sample": f"{steps1} steps ({wall1:.1f} s)"}
-    model = ""
+    out["host"] = {"cpu_count": os.cpu_count(), "model": cpu_model()}
+    return out
+
+
+def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+            return next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
     except OSError:
-        pass
-    out["host"] = {"cpu_count": os.cpu_count(), "model": model}
-    return out
+        return ""
 
 
 def main():
@@ -324,11 +357,18 @@ def main():
     ap.add_argument("--config3", type=int, default=0,
                     help="only run single FP32 runs of 2048^2 and 4096^2 for lambda0 in {2, 3.5, 5} for this "
                          "many steps (from the first step, energy on), one JSON line per run")
+    ap.add_argument("--cpu-configs", action="store_true",
+                    help="only time the CPU oracle on the configuration workloads (all host cores and one "
+                         "thread), one JSON line per leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(instances would shard across GPUs without communication).", file=sys.stderr)
         sys.exit(2)
+    if args.cpu_configs:
+        print(json.dumps({"host": {"cpu_count": os.cpu_count(), "model": cpu_model()}}), flush=True)
+        cpu_configs()
+        return
     if args.copy_calibration:
         import torch
         n = 64 << 20   # floats: 256 MiB per buffer
