@@ -77,6 +77,18 @@ def test_cli_run_and_continue(af, orc, tmp_path):
     assert "continue execution from snapshot 5" in r.stdout
     assert (d / "ActTurb_snap7.bin").exists()
     assert len((d / "energy.txt").read_text().splitlines()) == 50 + 21   # re-runs step 50 (.cu:612)
+    # resume parity against an uninterrupted run (.cu:612-613, 718-824): the
+    # continued loop starts at m = 5 * snap and steps once before it writes
+    # snapshot 5 again, so the continued snapshots 5 and 7 hold the
+    # uninterrupted states after 51 and 71 steps (17-digit snapshots: lossless)
+    o.step(1)
+    w5b, _ = af.snapshot_read(d / "ActTurb_snap5.bin", 64, 64)
+    assert rel_l2(w5b, o.get_state()[0]) < 1e-12
+    o.step(20)
+    w7, M7 = af.snapshot_read(d / "ActTurb_snap7.bin", 64, 64)
+    wo7, mo7 = o.get_state()
+    assert rel_l2(w7, wo7) < 1e-12
+    assert np.allclose(M7, mo7, rtol=1e-11, atol=1e-11)
 
 
 def test_cli_nan_exit(tmp_path):
