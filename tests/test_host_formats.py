@@ -189,6 +189,16 @@ def test_cli_argument_errors(tmp_path):
     assert r.returncode == 0 and "specifications.txt" in r.stdout
 
 
+def test_cli_math_writes_theory(tmp_path):
+    """`math` (.cu:270-315): a LaTeX file theory.tex in the working directory,
+    with the vorticity and mean-velocity equations; exit code 0."""
+    r = run_cli(["math"], tmp_path)
+    assert r.returncode == 0 and "theory.tex" in r.stdout
+    tex = (tmp_path / "theory.tex").read_text()
+    assert tex.startswith("\\documentclass{article}") and tex.rstrip().endswith("\\end{document}")
+    assert tex.count("\\begin{equation}") == 2 and "\\partial_t\\omega" in tex
+
+
 def test_cli_failure_injection(tmp_path):
     d = tmp_path / "sim"
     d.mkdir()
