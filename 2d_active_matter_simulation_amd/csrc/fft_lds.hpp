@@ -99,14 +99,20 @@ constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 //                     once Ns*W >= 16 (stores already hit 16 consecutive
 //                     elements);
 //   16-byte elements: Ns*W extra per Ns*R*W, none once Ns*W*16 >= 256.
-// Every store and load pair of every schedule is conflict-free under that
-// rule, and for a given exchange each lane's R stores and R loads sit at one
+// Radix 16 with W >= 2 is the exception at Ns = 1: its padded load offsets
+// are too far apart to pair, so the loads compile to single ds_read_b64,
+// which are served 32 lanes at a time (elements must differ mod 32); 2 extra
+// elements after every 32 keep those loads and the paired stores
+// conflict-free (1 after every 16 left a 2-way conflict on the loads).
+// Every store and load of every schedule is conflict-free under these
+// rules, and for a given exchange each lane's R stores and R loads sit at one
 // lane base plus lane-independent offsets (lds_pad of the offset alone), so
 // fft_multi computes one base per lane and uses the offsets as immediates.
 template <int W, int R, int ESZ>
 __host__ __device__ constexpr int lds_pad(int i, int Ns) {
   if constexpr (ESZ == 8) {
     if (Ns * W >= 16) return i;
+    if (R == 16 && Ns == 1 && W >= 2) return i + (i / 32) * 2;
     const int P = (Ns * W < 16 / R) ? Ns * W : 16 / R;
     return i + (i / 16) * P;
   } else {
@@ -175,6 +181,12 @@ struct FftMulti {
   static constexpr int kLastLoadSet = (RL > 1) ? (NP - 1) % 2 : (NP >= 2 ? (NP - 2) % 2 : -1);
   static constexpr int AREA = lds_area_elems<L, W, R>();
   static __device__ __forceinline__ int pad(int i, int Ns) { return lds_pad<W, R, ESZ>(i, Ns); }
+  static constexpr bool pads_fit() {
+    for (int ns = 1; ns < L; ns *= R)
+      if (lds_pad<W, R, ESZ>(L * W - 1, ns) >= AREA) return false;
+    return true;
+  }
+  static_assert(pads_fit(), "every padded exchange fits the LDS area");
 
   // loads of the positions j + r*S written by the pass of stride Nsp
   template <int Nsp>
