@@ -59,8 +59,17 @@ af_status af_engine_add(af_engine* e, const af_params* p, const double* omega_ha
   Geometry g = geometry(p, true);   // the engine's tuned table (batched launches)
   if (!g.rc || !g.cc) return fail(AF_EUNSUPPORTED, "no kernel for this size");
   if (e->in_use + g.bytes > e->budget) return fail(AF_ENOMEM, "HBM budget exceeded");
+  st = nyquist_check(g, omega_hat);
+  if (st != AF_OK) return st;
+  // the class stream first: nothing is claimed if it cannot be created
+  Class& c = e->classes[ClassKey(p->precision, p->nx, p->ny, g.sched)];
+  hipError_t he;
+  if (!c.stream) {
+    he = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+    if (he != hipSuccess) { c.stream = nullptr; return hip_fail(he, "class stream"); }
+  }
   unsigned char* base = nullptr;
-  hipError_t he = hipMalloc(&base, g.bytes);
+  he = hipMalloc(&base, g.bytes);
   if (he != hipSuccess) return fail(AF_ENOMEM, std::string("device allocation: ") + hipGetErrorString(he));
   st = init_device_block(p, g, base, e->stream);
   if (st == AF_OK) st = load_state(g, base, omega_hat, M, e->stream);
@@ -76,11 +85,6 @@ af_status af_engine_add(af_engine* e, const af_params* p, const double* omega_ha
   s.base = base;
   s.steps = 0;
   e->in_use += g.bytes;
-  Class& c = e->classes[key_of(s)];
-  if (!c.stream) {
-    he = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
-    if (he != hipSuccess) return hip_fail(he, "class stream");
-  }
   *slot = id;
   return AF_OK;
 }

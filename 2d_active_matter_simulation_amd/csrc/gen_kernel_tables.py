@@ -11,11 +11,14 @@ otherwise) and, for square grids from 256^2 to 4096^2, a few alternatives for
 the autotuner (autotune.py).  Constraints checked here: at most 512 lanes per
 block and at most 160 KB of LDS per block.
 
-    python3 gen_kernel_tables.py        # rewrites the two .hip files
+    python3 gen_kernel_tables.py OUTDIR   # writes the .hip files into OUTDIR
+
+The Makefile runs it into its build directory; the generated files are not
+kept in the source tree.
 """
 import os
+import sys
 
-HERE = os.path.dirname(os.path.abspath(__file__))
 SIZES = [32, 64, 128, 256, 512, 1024, 2048, 4096]
 LDS_MAX = 160 * 1024
 
@@ -163,7 +166,7 @@ ConvLaunch from_bricks_{tag}() {{ return launch_from_bricks<T>; }}
 """
 
 
-def gen(prec):
+def gen(prec, outdir):
     tag = "f32" if prec == 32 else "f64"
     T = "float" if prec == 32 else "double"
     scheds = []
@@ -203,14 +206,11 @@ def gen(prec):
     externs = "\n".join(f"extern template {sig};" for _, sig in insts)
     text = HDR.format(what=f"FP{prec}", T=T, tag=tag, rows="\n".join(rl), cols="\n".join(cl), scheds="\n".join(sl),
                       ifac=ifac, externs=externs)
-    with open(os.path.join(HERE, f"kernels_{tag}.hip"), "w") as f:
+    with open(os.path.join(outdir, f"kernels_{tag}.hip"), "w") as f:
         f.write(text)
-    for old in os.listdir(HERE):
-        if old.startswith(f"kernels_{tag}_k") and old.endswith(".hip"):
-            os.remove(os.path.join(HERE, old))
     for k, sigs in enumerate(parts):
         body = "\n".join(f"template {sig};" for sig in sigs)
-        with open(os.path.join(HERE, f"kernels_{tag}_k{k}.hip"), "w") as f:
+        with open(os.path.join(outdir, f"kernels_{tag}_k{k}.hip"), "w") as f:
             f.write(PART.format(what=f"FP{prec}", k=k, n=nparts, body=body))
     return len(rl), len(cl), len(sl)
 
@@ -227,5 +227,8 @@ namespace af {{
 
 
 if __name__ == "__main__":
+    if len(sys.argv) != 2:
+        sys.exit("usage: gen_kernel_tables.py OUTDIR")
+    os.makedirs(sys.argv[1], exist_ok=True)
     for prec in (32, 64):
-        print(prec, gen(prec))
+        print(prec, gen(prec, sys.argv[1]))

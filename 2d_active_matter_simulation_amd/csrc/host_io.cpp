@@ -15,7 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "run_impl.hpp"
+#include "status.hpp"
 
 using namespace af;
 

@@ -13,13 +13,6 @@
 
 namespace af {
 
-thread_local std::string g_last_error;
-
-af_status fail(af_status s, const std::string& msg) {
-  g_last_error = msg;
-  return s;
-}
-
 af_status hip_fail(hipError_t e, const char* what) {
   g_last_error = std::string(what) + ": " + hipGetErrorString(e);
   return AF_EDEVICE;
@@ -181,6 +174,20 @@ af_status init_device_block(const af_params* p, const Geometry& g, unsigned char
   return AF_OK;
 }
 
+// The kx = Nx/2 column is not stored on this backend (every supported mask
+// removes it from the nonlinear term).  A state that carries it would be
+// changed by dropping it, so such a state is refused instead.
+af_status nyquist_check(const Geometry& g, const double* omega_hat) {
+  const size_t nxh = (size_t)g.nx2 + 1;
+  for (int y = 0; y < g.ny; ++y) {
+    const double* v = omega_hat + 2 * ((size_t)y * nxh + g.nx2);
+    if (v[0] != 0.0 || v[1] != 0.0)
+      return fail(AF_EUNSUPPORTED, "the state has a nonzero kx = Nx/2 column (row " + std::to_string(y) +
+                                       "), which this backend does not represent; zero it to load the state");
+  }
+  return AF_OK;
+}
+
 // Reference-layout state -> bricks + mean, then the stage-0 column pass.
 af_status load_state(const Geometry& g, unsigned char* base, const double* omega_hat, const double M[2],
                      hipStream_t stream) {
@@ -254,11 +261,20 @@ hipError_t launch_step(const Geometry& g, const void* descs, int ninst, const vo
 
 using namespace af;
 
+// Host side of the per-chunk hand-back: the mean state (NaN flag, step count)
+// and the chunk's per-step energies, in pinned memory so the copies are
+// direct DMA and need no staging.
+struct HostChunk {
+  MeanState ms;
+  double energy[kRingCap];
+};
+
 struct af_run {
   af_params p;
   Geometry g;
   hipStream_t stream = nullptr;
   unsigned char* base = nullptr;
+  HostChunk* host = nullptr;        // pinned
   hipGraph_t graph = nullptr;       // one step
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph_n = nullptr;     // kGraphSteps steps
@@ -270,21 +286,6 @@ struct af_run {
 };
 
 extern "C" {
-
-const char* af_status_str(af_status s) {
-  switch (s) {
-    case AF_OK: return "ok";
-    case AF_EINVAL: return "invalid argument";
-    case AF_ENOMEM: return "out of memory";
-    case AF_EDEVICE: return "device error";
-    case AF_ENAN: return "NaN in the solution";
-    case AF_EIO: return "I/O error";
-    case AF_EUNSUPPORTED: return "unsupported on this backend";
-  }
-  return "unknown status";
-}
-
-const char* af_last_error(void) { return g_last_error.c_str(); }
 
 void af_params_reference(af_params* p, int nx, int ny, double lx, double ly, double dt, double lambda, double alpha,
                          double beta) {
@@ -323,17 +324,28 @@ af_status af_run_create(const af_params* p, af_run** out) {
     delete r;
     return fail(AF_ENOMEM, std::string("device allocation: ") + hipGetErrorString(e));
   }
+  e = hipHostMalloc((void**)&r->host, sizeof(HostChunk), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    r->host = nullptr;
+    af_run_destroy(r);
+    return fail(AF_ENOMEM, std::string("pinned host buffer: ") + hipGetErrorString(e));
+  }
   st = init_device_block(p, r->g, r->base, r->stream);
   if (st != AF_OK) { af_run_destroy(r); return st; }
   *out = r;
   return AF_OK;
 }
 
+static af_status prepare_graphs(af_run* r);
+
 af_status af_run_set_state(af_run* r, const double* omega_hat, const double M[2]) {
   if (!r || !omega_hat || !M) return fail(AF_EINVAL, "NULL argument");
-  af_status st = load_state(r->g, r->base, omega_hat, M, r->stream);
-  r->ready = (st == AF_OK);
+  r->ready = false;
   r->steps = 0;
+  af_status st = nyquist_check(r->g, omega_hat);
+  if (st == AF_OK) st = load_state(r->g, r->base, omega_hat, M, r->stream);
+  if (st == AF_OK) st = prepare_graphs(r);
+  r->ready = (st == AF_OK);
   return st;
 }
 
@@ -368,6 +380,23 @@ static bool capture_steps(af_run* r, int steps, hipGraph_t* graph, hipGraphExec_
   return true;
 }
 
+// Captures, instantiates and uploads both step graphs (once per run: they
+// depend only on the run's buffers, not on its state), so that the first
+// chunk of steps after set_state already replays resident graphs.  A failed
+// capture is not an error: the run then launches its kernels directly.
+static af_status prepare_graphs(af_run* r) {
+  if (!r->use_graph || r->graph_failed) return AF_OK;
+  if (!r->gexec && !capture_steps(r, 1, &r->graph, &r->gexec)) r->graph_failed = true;
+  if (!r->graph_failed && !r->gexec_n && !capture_steps(r, kGraphSteps, &r->graph_n, &r->gexec_n))
+    r->graph_failed = true;
+  if (r->graph_failed) return AF_OK;
+  hipError_t e = hipGraphUpload(r->gexec, r->stream);
+  if (e == hipSuccess) e = hipGraphUpload(r->gexec_n, r->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+  if (e != hipSuccess) return hip_fail(e, "graph upload");
+  return AF_OK;
+}
+
 // Enqueues `steps` (1 or kGraphSteps) steps: as a captured graph, or as plain
 // launches when graphs are off or could not be captured.
 static hipError_t run_steps(af_run* r, int steps) {
@@ -391,24 +420,35 @@ af_status af_run_step(af_run* r, int nsteps, double* energy, long long* nan_step
   int done = 0;
   while (done < nsteps) {
     const int k = (nsteps - done) < kRingCap ? (nsteps - done) : kRingCap;
-    for (int i = 0; i < k;) {
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < k && e == hipSuccess;) {
       const int n = (k - i >= kGraphSteps) ? kGraphSteps : 1;
-      hipError_t e = run_steps(r, n);
-      if (e != hipSuccess) return hip_fail(e, "step launch");
+      e = run_steps(r, n);
       i += n;
     }
-    MeanState ms;
-    hipError_t e = hipMemcpyAsync(&ms, r->base + r->g.off_mean, sizeof(ms), hipMemcpyDeviceToHost, r->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
-    if (e != hipSuccess) return hip_fail(e, "step sync");
-    if (energy) {
-      std::vector<double> ring(kRingCap);
-      e = hipMemcpy(ring.data(), r->base + r->g.off_energy, kRingCap * sizeof(double), hipMemcpyDeviceToHost);
-      if (e != hipSuccess) return hip_fail(e, "energy download");
-      for (int i = 0; i < k; ++i) energy[done + i] = ring[(size_t)((r->steps + i) % kRingCap)];
+    // the chunk's mean state and its k energies (ring entries steps .. steps+k-1,
+    // at most two contiguous pieces) into pinned memory, one synchronisation
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(&r->host->ms, r->base + r->g.off_mean, sizeof(MeanState), hipMemcpyDeviceToHost, r->stream);
+    if (e == hipSuccess && energy) {
+      const int first = (int)(r->steps % kRingCap);
+      const int n1 = k < kRingCap - first ? k : kRingCap - first;
+      const double* ring = reinterpret_cast<const double*>(r->base + r->g.off_energy);
+      e = hipMemcpyAsync(r->host->energy, ring + first, n1 * sizeof(double), hipMemcpyDeviceToHost, r->stream);
+      if (e == hipSuccess && k > n1)
+        e = hipMemcpyAsync(r->host->energy + n1, ring, (k - n1) * sizeof(double), hipMemcpyDeviceToHost, r->stream);
     }
+    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+    if (e != hipSuccess) {
+      // some steps of the chunk may have run: the host step count no longer
+      // matches the device's, so the run needs a new set_state
+      r->ready = false;
+      return hip_fail(e, "step");
+    }
+    if (energy) std::memcpy(energy + done, r->host->energy, (size_t)k * sizeof(double));
     r->steps += k;
     done += k;
+    const MeanState& ms = r->host->ms;
     if (ms.nan_step >= 0) {
       if (nan_step) *nan_step = ms.nan_step;
       return fail(AF_ENAN, "array contains nan values after " + std::to_string(ms.nan_step) + " timesteps");
@@ -520,6 +560,7 @@ void af_run_destroy(af_run* r) {
   if (r->gexec_n) (void)hipGraphExecDestroy(r->gexec_n);
   if (r->graph_n) (void)hipGraphDestroy(r->graph_n);
   if (r->base) (void)hipFree(r->base);
+  if (r->host) (void)hipHostFree(r->host);
   if (r->stream) (void)hipStreamDestroy(r->stream);
   delete r;
 }

@@ -8,6 +8,7 @@
 #include "../../include/actflow.h"
 #include "layout.hpp"
 #include "registry.hpp"
+#include "status.hpp"
 
 namespace af {
 
@@ -30,12 +31,11 @@ struct Geometry {
   size_t bytes = 0;
 };
 
-extern thread_local std::string g_last_error;
-af_status fail(af_status s, const std::string& msg);
 af_status hip_fail(hipError_t e, const char* what);
 af_status validate(const af_params* p);
 Geometry geometry(const af_params* p, bool batched = false);
 af_status init_device_block(const af_params* p, const Geometry& g, unsigned char* base, hipStream_t stream);
+af_status nyquist_check(const Geometry& g, const double* omega_hat);
 af_status load_state(const Geometry& g, unsigned char* base, const double* omega_hat, const double M[2],
                      hipStream_t stream);
 af_status store_state(const Geometry& g, unsigned char* base, double* omega_hat, double M[2], hipStream_t stream);

@@ -80,16 +80,42 @@ def read_traffic(workload_tag):
     return vals or None
 
 
+def rel_l2(a, b):
+    """Real-space relative L2 of two half spectra (Parseval weights)."""
+    wt = np.full(a.shape, 2.0)
+    wt[:, 0] = 1.0
+    wt[:, -1] = 1.0
+    return float(np.sqrt(np.sum(wt * np.abs(a - b) ** 2) / np.sum(wt * np.abs(b) ** 2)))
+
+
+def fp32_error(af, p32, ic, M0, steps, w32):
+    """Relative L2 on omega of an FP32 result against the same run in FP64
+    (the FP64 GPU path, equal to the FP64 oracle to <= 1e-12 per step in the
+    parity tests), after the same number of steps from the same state."""
+    q = af.AfParams.from_buffer_copy(p32)
+    q.precision = 64
+    run = af.Run(q)
+    run.set_state(ic, M0)
+    run.step(steps)
+    w64, _ = run.get_state()
+    run.close()
+    return {"fp32_rel_l2_vs_fp64": rel_l2(w32, w64), "after_steps": int(steps)}
+
+
 def run_workload(af, n, prec, steps, warmup, seed=1, host_io=False):
     p = af.params(n, dt=1e-4 if n <= 1024 else 5e-5, precision=prec, **PHYS)
+    ic = af.ic_bandlimited(n, seed=seed, amp=1e-3, kmax=10.0)
     run = af.Run(p)
-    run.set_state(af.ic_bandlimited(n, seed=seed, amp=1e-3, kmax=10.0))
+    run.set_state(ic)
     if warmup:
         run.step(warmup)
     t0 = time.perf_counter()
     e = run.step(steps, energy=True)          # returns after the device finished
     wall = time.perf_counter() - t0
     assert np.all(np.isfinite(e))
+    err = None
+    if prec == 32:
+        err = fp32_error(af, p, ic, (0.0, 0.0), warmup + steps, run.get_state()[0])
     io = None
     if host_io:
         # the C ABI hands the state over in host memory (FP64 half spectrum):
@@ -110,7 +136,7 @@ def run_workload(af, n, prec, steps, warmup, seed=1, host_io=False):
     run.close()
     return dict(n=n, prec=prec, steps=steps, wall=wall, ms_per_step=wall / steps * 1e3,
                 value=n * n * steps / wall, row_ms=row_ms, col_ms=col_ms, prof_steps=kprof,
-                prof_ms_per_step=tot_ms / kprof, host_io=io)
+                prof_ms_per_step=tot_ms / kprof, host_io=io, fp32_error=err)
 
 
 def roofline(af, r):
@@ -164,6 +190,12 @@ def ensemble_leg(af, steps=200, seed=7, measured_sequential=False):
         # the engine's batched schedules differ from the single-run ones, so the
         # two agree to FP32 rounding grown over the steps, not bitwise
         out["max_rel_diff_engine_vs_alone"] = worst
+    # FP32 error of the first instance of each size class against FP64
+    first = {}
+    for slot, p, ic in runs:
+        first.setdefault(p.nx, (slot, p, ic))
+    out["fp32_error"] = {f"{n}^2": fp32_error(af, p, ic, (0.0, 0.0), steps + 10, eng.get_state(slot)[0])
+                         for n, (slot, p, ic) in first.items()}
     eng.close()
     if not measured_sequential:
         seq = 0.0
@@ -191,15 +223,17 @@ def config3_leg(af, steps=5000):
     for n in (2048, 4096):
         for lam in (2.0, 3.5, 5.0):
             p = af.params(n, dt=5e-5, lam=lam, alpha=-0.8, beta=0.01, gamma0=-0.045, gamma2=2.5e-5, precision=32)
+            ic = af.ic_bandlimited(n, seed=1, amp=1e-3, kmax=10.0)
             run = af.Run(p)
-            run.set_state(af.ic_bandlimited(n, seed=1, amp=1e-3, kmax=10.0))
+            run.set_state(ic)
             t0 = time.perf_counter()
             e = run.step(steps, energy=True)
             wall = time.perf_counter() - t0
+            err = fp32_error(af, p, ic, (0.0, 0.0), steps, run.get_state()[0])
             run.close()
             out.append({"grid": [n, n], "lambda0": lam, "steps": steps, "value": n * n * steps / wall,
                         "ms_per_step": wall / steps * 1e3, "energy_first": float(e[0]), "energy_last": float(e[-1]),
-                        "finite": bool(np.all(np.isfinite(e)))})
+                        "finite": bool(np.all(np.isfinite(e))), "fp32_error": err})
             print(json.dumps({"config3": out[-1]}), flush=True)
     return out
 
@@ -209,18 +243,22 @@ def batched_leg(af, n, count, steps, seed=3):
     lambda0 cycling over {2, 3.5, 5} (other parameters as the headline), in
     shared launches of the ensemble engine (its tuned batched schedule)."""
     eng = af.Engine(200 << 30)
+    slots = []
     for i in range(count):
         p = af.params(n, dt=1e-4 if n <= 1024 else 5e-5, lam=(2.0, 3.5, 5.0)[i % 3], alpha=-0.8, beta=0.01,
                       gamma0=-0.045, gamma2=2.5e-5, precision=32)
-        eng.add(p, af.ic_bandlimited(n, seed=seed + i, amp=1e-3, kmax=10.0))
+        ic = af.ic_bandlimited(n, seed=seed + i, amp=1e-3, kmax=10.0)
+        slots.append((eng.add(p, ic), p, ic))
     eng.step(5)
     t0 = time.perf_counter()
     eng.step(steps)
     wall = time.perf_counter() - t0
+    slot, p, ic = slots[0]
+    err = fp32_error(af, p, ic, (0.0, 0.0), steps + 5, eng.get_state(slot)[0])
     eng.close()
     fb = field_bytes(n, 32) * count
     return {"grid": [n, n], "instances": count, "steps": steps, "value": count * n * n * steps / wall,
-            "ms_per_step": wall / steps * 1e3,
+            "ms_per_step": wall / steps * 1e3, "fp32_error": {"instance 0": err},
             "hbm_frac_50_transfers": 50 * fb / (wall / steps) / 1e9 / HBM_PEAK_GBS,
             "step_model_frac": count * step_model_bytes(n) / (wall / steps) / 1e9 / HBM_PEAK_GBS}
 
@@ -414,6 +452,8 @@ def main():
                                       "frac": ach / HBM_PEAK_GBS,
                                       "step_model_frac": step_model_bytes(n, prec)
                                       / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            if r["fp32_error"]:
+                extra[f"{n}_fp{prec}"]["fp32_error"] = r["fp32_error"]
         if N == 1024:
             extra["fp32_over_fp64_1024"] = head["value"] / extra["1024_fp64"]["value"]
     out = {
@@ -439,6 +479,7 @@ def main():
                      "algorithmic_bytes_per_launch": alg, "avg_launch_ms": kms,
                      "events_over_steps": head["prof_steps"],
                      "step_model_frac": step_model_bytes(N) / (head["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "fp32_error": head["fp32_error"],
         "extra": extra,
     }
     io = head["host_io"]

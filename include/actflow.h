@@ -74,7 +74,10 @@ af_status af_run_create(const af_params* p, af_run** out);
 /* Replaces the state load of .cu:615-634 / the IC hand-off of .cu:660-679.
  * omega_hat: 2*Ny*(Nx/2+1) doubles (reference layout).  M: 2 doubles.
  * The kx = Nx/2 column is not represented on this backend (every supported
- * mask removes it from the nonlinear term); it is taken as zero. */
+ * mask removes it from the nonlinear term): a state whose kx = Nx/2 column
+ * is nonzero is refused with AF_EUNSUPPORTED (dropping it would change the
+ * run); zero that column to load such a state.  Also captures and uploads
+ * the run's step graphs, so the first af_run_step replays resident graphs. */
 af_status af_run_set_state(af_run* r, const double* omega_hat, const double M[2]);
 /* Replaces the device->host copy of save_func (.cu:1259-1260). */
 af_status af_run_get_state(af_run* r, double* omega_hat, double M[2]);

@@ -2,8 +2,6 @@
 20k steps): per-step relative L2 of omega for the first 200 steps, and the
 time averages of energy, enstrophy and the shell spectrum over steps
 10k-20k with batch-means error bars (10 batches)."""
-import os
-
 import numpy as np
 import pytest
 
@@ -105,25 +103,24 @@ def test_long_run_fp32_vs_fp64_statistics(af):
         assert np.all(np.abs(S32 - S64)[band] <= 4 * sig[band] + 0.02 * S64[band])
 
 
-@pytest.mark.skipif(os.environ.get("AF_LONG_STATS") != "1",
-                    reason="long run (about 3 minutes of GPU time): set AF_LONG_STATS=1")
 @pytest.mark.timeout(900)
 def test_long_run_energy_within_one_percent(af):
     """Stationary window: 1024^2, dt = 1e-4, two seeds, 300k steps (t = 30),
-    averaged over steps 20k-300k.  FP32 runs go through the ensemble engine
-    (the batched form, both seeds in one launch per pass); FP64 runs are
-    plain single runs, the GPU path that equals the oracle to 1e-12 per step.
-    Energies come from the per-step energy series (means of 100 steps),
-    enstrophy and E(k) from the diagnostics every 1000 steps; error bars are
-    batch means (10 batches per seed) combined over the two seeds."""
+    averaged over steps 20k-300k (about 2 minutes of GPU time).  All four
+    runs (FP32 and FP64, two seeds) sit in one ensemble engine, so FP32 runs
+    in the batched form and the two precisions execute concurrently; the FP64
+    runs take the same fused kernels as the plain FP64 path, which equals the
+    oracle to 1e-12 per step.  Energies come from the per-step energy series
+    (means of 100 steps), enstrophy and E(k) from the diagnostics every 1000
+    steps; error bars are batch means (10 batches per seed) combined over the
+    two seeds."""
     N, dt, total, start, chunk = 1024, 1e-4, 300000, 20000, 1000
     seeds = (1, 2)
     eng = af.Engine(8 << 30)
-    slots = [eng.add(af.params(N, dt=dt, precision=32, **JW), af.ic_bandlimited(N, seed=s, amp=1e-3, kmax=10.0))
-             for s in seeds]
-    runs = [af.Run(af.params(N, dt=dt, precision=64, **JW)) for _ in seeds]
-    for r, s in zip(runs, seeds):
-        r.set_state(af.ic_bandlimited(N, seed=s, amp=1e-3, kmax=10.0))
+    s32 = [eng.add(af.params(N, dt=dt, precision=32, **JW), af.ic_bandlimited(N, seed=s, amp=1e-3, kmax=10.0))
+           for s in seeds]
+    s64 = [eng.add(af.params(N, dt=dt, precision=64, **JW), af.ic_bandlimited(N, seed=s, amp=1e-3, kmax=10.0))
+           for s in seeds]
     e32 = [[] for _ in seeds]
     e64 = [[] for _ in seeds]
     d32 = [[] for _ in seeds]
@@ -131,19 +128,17 @@ def test_long_run_energy_within_one_percent(af):
     done = 0
     while done < total:
         eng.step(chunk)
-        for i, r in enumerate(runs):
-            e64[i].append(r.step(chunk, energy=True).reshape(-1, 100).mean(axis=1))
-            e32[i].append(eng.energy(slots[i], chunk).reshape(-1, 100).mean(axis=1))
+        for i in range(len(seeds)):
+            e32[i].append(eng.energy(s32[i], chunk).reshape(-1, 100).mean(axis=1))
+            e64[i].append(eng.energy(s64[i], chunk).reshape(-1, 100).mean(axis=1))
         done += chunk
         if done > start:
-            for i, r in enumerate(runs):
-                d32[i].append(eng.diagnostics(slots[i], nshell=64))
-                d64[i].append(r.diagnostics(nshell=64))
+            for i in range(len(seeds)):
+                d32[i].append(eng.diagnostics(s32[i], nshell=64))
+                d64[i].append(eng.diagnostics(s64[i], nshell=64))
         if done % 50000 == 0:
             print(f"step {done}", flush=True)
     eng.close()
-    for r in runs:
-        r.close()
 
     def combine(per_seed):
         m = np.mean([p[0] for p in per_seed], axis=0)

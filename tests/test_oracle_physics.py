@@ -247,6 +247,96 @@ def test_projection_restores_hermitian_column(orc):
     assert asym < 1e-10 * np.abs(col).max()
 
 
+def broken_column_state(nx, ny, seed=3, amp=0.5, kmax=6):
+    """A state whose kx = 0 column is NOT Hermitian (rows y and Ny - y are
+    independent), the rest band-limited (|k| <= kmax), the kx = Nx/2 column
+    zero and the k = 0 mode zero."""
+    rng = np.random.default_rng(seed)
+    y = np.arange(ny)[:, None]
+    x = np.arange(nx // 2 + 1)[None, :]
+    yy = np.minimum(y, ny - y)
+    band = (x * x + yy * yy <= kmax * kmax) & (x < nx // 2)
+    band[0, 0] = False
+    noise = rng.standard_normal((ny, nx // 2 + 1)) + 1j * rng.standard_normal((ny, nx // 2 + 1))
+    w = np.where(band, amp * nx * ny * noise, 0.0).astype(complex)
+    # kx = 0: rows y and Ny - y drawn independently above; the Nyquist row
+    # (left alone by the projection) gets a value of its own
+    w[ny // 2, 0] = 0.25 * nx * ny * (1.0 - 2.0j)
+    return w
+
+
+def projection_closed_form(w0, E):
+    """One step with lambda = beta = 0 from w0: every stage's N-hat is 0, so
+    the step is w1 = P(E w0), P the projection of reality_func
+    (.cu:1237-1242): for y = 1..Ny/2-1, a = (x[y] + conj x[Ny-y]) / 2,
+    x[y] = a, x[Ny-y] = conj a; row 0, row Ny/2 and every kx > 0 are E w0."""
+    ny = w0.shape[0]
+    x = E * w0
+    out = x.copy()
+    for y in range(1, ny // 2):
+        a = 0.5 * (x[y, 0] + np.conj(x[ny - y, 0]))
+        out[y, 0] = a
+        out[ny - y, 0] = np.conj(a)
+    return out
+
+
+@pytest.mark.parametrize("nx,ny", [(32, 32), (32, 64)])
+def test_projection_value_closed_form(orc, nx, ny):
+    """The projection's value, not only its symmetry: from a kx = 0 column
+    with broken symmetry and lambda = beta = 0, one step gives exactly
+    P(E w0).  A copy instead of an average, a wrong partner row, or a
+    projected Nyquist row each fail here."""
+    dt, alpha = 1e-2, -0.8
+    o = orc.Oracle(nx, ny, lam=0.0, beta=0.0, dt=dt, alpha=alpha, dealias=2)
+    w0 = broken_column_state(nx, ny)
+    o.set_state(w0, (0.0, 0.0))
+    o.step(1)
+    w, _ = o.get_state()
+    kx = np.arange(nx // 2 + 1)[None, :]
+    ky = np.fft.fftfreq(ny, 1.0 / ny)[:, None]
+    k2 = kx ** 2 + ky ** 2
+    E = np.exp((-(1 - k2) ** 2 - alpha) * dt)
+    ref = projection_closed_form(w0, E)
+    scale = np.abs(ref).max()
+    assert np.abs(w - ref).max() <= 1e-14 * scale
+    # the symmetry was really broken: without the projection the column differs
+    assert np.abs(E * w0 - ref)[:, 0].max() > 1e-3 * scale
+    assert w[ny // 2, 0] == E[ny // 2, 0] * w0[ny // 2, 0]
+
+
+def _mean_closed_form_error(orc, dt, T=2.0, N=32, beta=0.5, scheme=0):
+    """sigma(0) = 0 (alpha = -1 in the reference operator: c0 = -1 - alpha = 0),
+    omega = 0: the mean obeys dM/dt = -beta |M|^2 M / N^4 (N^4 = (Nx Ny)^2,
+    beta' = beta/N^3 of .cu:434 times the N of the k=0 DFT), whose solution is
+    |M(t)|^2 = |M0|^2 / (1 + 2 beta |M0|^2 t / N^4) along M0.  Returns the
+    relative error of the oracle's M at T."""
+    o = orc.Oracle(N, lam=2.0, beta=beta, dt=dt, alpha=-1.0, mean_scheme=scheme)
+    M0 = np.array([0.9, -0.3]) * N * N
+    o.set_state(np.zeros((N, N // 2 + 1), complex), M0)
+    o.step(int(round(T / dt)))
+    w, M = o.get_state()
+    assert np.abs(w).max() < 1e-9 * N * N
+    m02 = np.dot(M0, M0)
+    ref = M0 * np.sqrt(1.0 / (1.0 + 2.0 * beta * m02 * T / (N * N) ** 2))
+    return np.linalg.norm(M - ref) / np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_mean_ode_is_classical_rk4(orc, scheme):
+    """With sigma(0) = 0 the reference's mean update (.cu:729-732, 756-757,
+    778-779, 799-800, 820-821) is classical RK4 with the stage inputs of
+    .cu:758-759, 780-781, 801-802: the error against the closed form falls
+    ~16x per halving of dt (measured 14.4 and 15.3: the cubic's error
+    constant reaches its asymptote slowly, so the steps are small).  A wrong
+    accumulate weight (e.g. dt/3 for dt/6) leaves an O(dt) error that fails
+    the bound; a wrong stage input drops the order to 1 or 2 and fails the
+    ratio."""
+    e = [_mean_closed_form_error(orc, dt, scheme=scheme) for dt in (0.05, 0.025, 0.0125)]
+    assert e[0] < 1e-8, e
+    r1, r2 = e[0] / e[1], e[1] / e[2]
+    assert 13.0 < r1 < 19.0 and 13.0 < r2 < 19.0, (e, r1, r2)
+
+
 def test_invariants_w0_and_mask(orc):
     """omega^[0] stays 0 and masked-out modes never gain content (T12)."""
     N = 32
