@@ -261,13 +261,21 @@ hipError_t launch_step(const Geometry& g, const void* descs, int ninst, const vo
 
 using namespace af;
 
-// Host side of the per-chunk hand-back: the mean state (NaN flag, step count)
-// and the chunk's per-step energies, in pinned memory so the copies are
-// direct DMA and need no staging.
+// Step graphs of 1, 2, 4, 8 and 16 steps: a chunk of k steps replays the
+// binary decomposition of k (16-step graphs, then at most one of each smaller
+// size), so there are few gaps between graph launches at any k.
+constexpr int kGraphLevels = 5;
+constexpr int kGraphSteps = 1 << (kGraphLevels - 1);
+
+// Host side of the per-chunk hand-back, mirroring the run's device block from
+// its mean state (NaN flag, step count) to the end of its energy ring, in
+// pinned memory: one direct DMA copy per chunk.
 struct HostChunk {
   MeanState ms;
+  unsigned char pad[256 - sizeof(MeanState)];
   double energy[kRingCap];
 };
+static_assert(sizeof(MeanState) <= 256, "the mean state fits its 256-byte slot");
 
 struct af_run {
   af_params p;
@@ -275,10 +283,8 @@ struct af_run {
   hipStream_t stream = nullptr;
   unsigned char* base = nullptr;
   HostChunk* host = nullptr;        // pinned
-  hipGraph_t graph = nullptr;       // one step
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph_n = nullptr;     // kGraphSteps steps
-  hipGraphExec_t gexec_n = nullptr;
+  hipGraph_t graph[kGraphLevels] = {};       // graph[l]: 2^l steps
+  hipGraphExec_t gexec[kGraphLevels] = {};
   bool use_graph = true;
   bool graph_failed = false;
   bool ready = false;
@@ -316,6 +322,10 @@ af_status af_run_create(const af_params* p, af_run** out) {
   r->p = *p;
   r->g = geometry(p);
   if (!r->g.rc || !r->g.cc) { delete r; return fail(AF_EUNSUPPORTED, "no kernel for this size"); }
+  if (r->g.off_energy - r->g.off_mean != offsetof(HostChunk, energy)) {   // layout HostChunk mirrors
+    delete r;
+    return fail(AF_EDEVICE, "internal: energy ring does not follow the mean state");
+  }
   hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete r; return hip_fail(e, "stream create"); }
   e = hipMalloc(&r->base, r->g.bytes);
@@ -355,10 +365,6 @@ af_status af_run_get_state(af_run* r, double* omega_hat, double M[2]) {
   return store_state(r->g, r->base, omega_hat, M, r->stream);
 }
 
-// Steps per multi-step graph: replaying one graph of 16 steps instead of 16
-// one-step graphs removes the gaps between graph launches.
-constexpr int kGraphSteps = 16;
-
 // Captures `steps` steps into (graph, exec); false (and nothing kept) on failure.
 static bool capture_steps(af_run* r, int steps, hipGraph_t* graph, hipGraphExec_t* exec) {
   const void* desc = r->base + r->g.off_desc;
@@ -386,24 +392,23 @@ static bool capture_steps(af_run* r, int steps, hipGraph_t* graph, hipGraphExec_
 // capture is not an error: the run then launches its kernels directly.
 static af_status prepare_graphs(af_run* r) {
   if (!r->use_graph || r->graph_failed) return AF_OK;
-  if (!r->gexec && !capture_steps(r, 1, &r->graph, &r->gexec)) r->graph_failed = true;
-  if (!r->graph_failed && !r->gexec_n && !capture_steps(r, kGraphSteps, &r->graph_n, &r->gexec_n))
-    r->graph_failed = true;
+  for (int l = 0; l < kGraphLevels && !r->graph_failed; ++l)
+    if (!r->gexec[l] && !capture_steps(r, 1 << l, &r->graph[l], &r->gexec[l])) r->graph_failed = true;
   if (r->graph_failed) return AF_OK;
-  hipError_t e = hipGraphUpload(r->gexec, r->stream);
-  if (e == hipSuccess) e = hipGraphUpload(r->gexec_n, r->stream);
+  hipError_t e = hipSuccess;
+  for (int l = 0; l < kGraphLevels && e == hipSuccess; ++l) e = hipGraphUpload(r->gexec[l], r->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
   if (e != hipSuccess) return hip_fail(e, "graph upload");
   return AF_OK;
 }
 
-// Enqueues `steps` (1 or kGraphSteps) steps: as a captured graph, or as plain
-// launches when graphs are off or could not be captured.
-static hipError_t run_steps(af_run* r, int steps) {
+// Enqueues 2^level steps: as a captured graph, or as plain launches when
+// graphs are off or could not be captured.
+static hipError_t run_steps(af_run* r, int level) {
+  const int steps = 1 << level;
   if (r->use_graph && !r->graph_failed) {
-    hipGraph_t* g = steps == 1 ? &r->graph : &r->graph_n;
-    hipGraphExec_t* x = steps == 1 ? &r->gexec : &r->gexec_n;
-    if (*x || capture_steps(r, steps, g, x)) return hipGraphLaunch(*x, r->stream);
+    if (r->gexec[level] || capture_steps(r, steps, &r->graph[level], &r->gexec[level]))
+      return hipGraphLaunch(r->gexec[level], r->stream);
     r->graph_failed = true;
   }
   const void* desc = r->base + r->g.off_desc;
@@ -421,23 +426,16 @@ af_status af_run_step(af_run* r, int nsteps, double* energy, long long* nan_step
   while (done < nsteps) {
     const int k = (nsteps - done) < kRingCap ? (nsteps - done) : kRingCap;
     hipError_t e = hipSuccess;
-    for (int i = 0; i < k && e == hipSuccess;) {
-      const int n = (k - i >= kGraphSteps) ? kGraphSteps : 1;
-      e = run_steps(r, n);
-      i += n;
-    }
-    // the chunk's mean state and its k energies (ring entries steps .. steps+k-1,
-    // at most two contiguous pieces) into pinned memory, one synchronisation
+    for (int i = 0; i + kGraphSteps <= k && e == hipSuccess; i += kGraphSteps) e = run_steps(r, kGraphLevels - 1);
+    for (int l = kGraphLevels - 2; l >= 0 && e == hipSuccess; --l)
+      if (k & (1 << l)) e = run_steps(r, l);
+    // the chunk's mean state and, if asked for, the energy ring (the k new
+    // entries are steps .. steps+k-1 mod the ring size): one copy into pinned
+    // memory of the mean state alone or of the block from it to the ring's end
+    // (the ring follows the mean state's 256-byte slot), one synchronisation
     if (e == hipSuccess)
-      e = hipMemcpyAsync(&r->host->ms, r->base + r->g.off_mean, sizeof(MeanState), hipMemcpyDeviceToHost, r->stream);
-    if (e == hipSuccess && energy) {
-      const int first = (int)(r->steps % kRingCap);
-      const int n1 = k < kRingCap - first ? k : kRingCap - first;
-      const double* ring = reinterpret_cast<const double*>(r->base + r->g.off_energy);
-      e = hipMemcpyAsync(r->host->energy, ring + first, n1 * sizeof(double), hipMemcpyDeviceToHost, r->stream);
-      if (e == hipSuccess && k > n1)
-        e = hipMemcpyAsync(r->host->energy + n1, ring, (k - n1) * sizeof(double), hipMemcpyDeviceToHost, r->stream);
-    }
+      e = hipMemcpyAsync(r->host, r->base + r->g.off_mean, energy ? sizeof(HostChunk) : sizeof(MeanState),
+                         hipMemcpyDeviceToHost, r->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
     if (e != hipSuccess) {
       // some steps of the chunk may have run: the host step count no longer
@@ -445,7 +443,8 @@ af_status af_run_step(af_run* r, int nsteps, double* energy, long long* nan_step
       r->ready = false;
       return hip_fail(e, "step");
     }
-    if (energy) std::memcpy(energy + done, r->host->energy, (size_t)k * sizeof(double));
+    if (energy)
+      for (int i = 0; i < k; ++i) energy[done + i] = r->host->energy[(r->steps + i) % kRingCap];
     r->steps += k;
     done += k;
     const MeanState& ms = r->host->ms;
@@ -555,10 +554,10 @@ af_status af_schedule_active(int precision, int nx, int ny, af_schedule* out) {
 void af_run_destroy(af_run* r) {
   if (!r) return;
   if (r->stream) (void)hipStreamSynchronize(r->stream);
-  if (r->gexec) (void)hipGraphExecDestroy(r->gexec);
-  if (r->graph) (void)hipGraphDestroy(r->graph);
-  if (r->gexec_n) (void)hipGraphExecDestroy(r->gexec_n);
-  if (r->graph_n) (void)hipGraphDestroy(r->graph_n);
+  for (int l = 0; l < kGraphLevels; ++l) {
+    if (r->gexec[l]) (void)hipGraphExecDestroy(r->gexec[l]);
+    if (r->graph[l]) (void)hipGraphDestroy(r->graph[l]);
+  }
   if (r->base) (void)hipFree(r->base);
   if (r->host) (void)hipHostFree(r->host);
   if (r->stream) (void)hipStreamDestroy(r->stream);
