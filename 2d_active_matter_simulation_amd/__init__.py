@@ -112,6 +112,7 @@ def lib() -> ctypes.CDLL:
         L.af_run_destroy.restype = None
         L.af_run_profile.argtypes = [vp, ctypes.c_int, dp, dp, dp]
         L.af_run_diagnostics.argtypes = [vp, dp, dp, dp, ctypes.c_int]
+        L.af_run_diagnostics_noalloc.argtypes = [vp, dp, dp, dp, ctypes.c_int]
         L.af_engine_diagnostics.argtypes = [vp, ctypes.c_int, dp, dp, dp, ctypes.c_int]
         L.af_run_use_graph.argtypes = [vp, ctypes.c_int]
         L.af_run_device_bytes.argtypes = [ctypes.POINTER(AfParams)]
@@ -233,6 +234,15 @@ class Run:
         spec = np.zeros(max(nshell, 1))
         _check(lib().af_run_diagnostics(self._h, ctypes.byref(e), ctypes.byref(z),
                                         _dp(spec) if nshell else None, nshell))
+        return e.value, z.value, (spec if nshell else None)
+
+    def diagnostics_noalloc(self, nshell: int = 0):
+        """As diagnostics(), through the allocation-free entry point (static
+        per-device scratch, nshell <= 4096)."""
+        e, z = ctypes.c_double(), ctypes.c_double()
+        spec = np.zeros(max(nshell, 1))
+        _check(lib().af_run_diagnostics_noalloc(self._h, ctypes.byref(e), ctypes.byref(z),
+                                                _dp(spec) if nshell else None, nshell))
         return e.value, z.value, (spec if nshell else None)
 
     def use_graph(self, on: bool):

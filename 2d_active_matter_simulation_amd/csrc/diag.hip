@@ -6,6 +6,8 @@
 // additions not fixed, ~1e-16 relative).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "run_impl.hpp"
@@ -51,15 +53,17 @@ __global__ void __launch_bounds__(256) diag_kernel(const cx<T>* __restrict__ w, 
   }
 }
 
+constexpr int kDiagBlocks = 512;
+
 af_status diagnostics(const af_params& p, const Geometry& g, unsigned char* base, hipStream_t stream, double* energy,
                       double* enstrophy, double* spectrum, int nshell) {
-  const int blocks = 512;
+  const int blocks = kDiagBlocks;
   const size_t bytes = (size_t)2 * blocks * sizeof(double) + (size_t)(nshell > 0 ? nshell : 1) * sizeof(double);
-  double* scratch = nullptr;
-  hipError_t e = hipMalloc(&scratch, bytes);
+  double* buf = nullptr;
+  hipError_t e = hipMalloc(&buf, bytes);
   if (e != hipSuccess) return fail(AF_ENOMEM, "diagnostics scratch");
-  double* part = scratch;
-  double* spec = scratch + 2 * blocks;
+  double* part = buf;
+  double* spec = buf + 2 * blocks;
   const double twopi = 6.283185307179586476925286766559;
   const double tkx = twopi / p.lx, tky = twopi / p.ly;
   const double kunit = tkx < tky ? tkx : tky;
@@ -98,8 +102,104 @@ af_status diagnostics(const af_params& p, const Geometry& g, unsigned char* base
       spectrum[0] += emean;
     }
   } while (0);
-  (void)hipFree(scratch);
+  (void)hipFree(buf);
   return st;
+}
+
+// ---- allocation-free variant --------------------------------------------
+// Device memory of af_run_diagnostics_noalloc: the block partials, the two
+// results and up to kNoAllocMaxShells spectrum bins, in one static array per
+// device (40 KiB), held until the process ends.  Its address is looked up
+// once per device (hipGetSymbolAddress, that device current) and only that
+// address is passed to kernels and copies.  Every call on a device shares the
+// array: overlapping calls on one device race.
+constexpr int kNoAllocMaxShells = 4096;
+constexpr int kScratchOut = 2 * kDiagBlocks;     // energy, enstrophy
+constexpr int kScratchSpec = kScratchOut + 2;    // spectrum bins
+static __device__ __align__(256) double scratch[kScratchSpec + kNoAllocMaxShells];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices];
+
+// The address of `scratch` on device dev (looked up once, with dev current).
+static af_status scratch_on(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(AF_EDEVICE, "device index beyond the static scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int prev = -1;
+    hipError_t e = hipGetDevice(&prev);
+    if (e == hipSuccess && prev != dev) e = hipSetDevice(dev);
+    void* ptr = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (prev >= 0 && prev != dev) (void)hipSetDevice(prev);
+    if (e != hipSuccess) return hip_fail(e, "scratch address");
+    if (!ptr) return fail(AF_EDEVICE, "scratch address is NULL");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return AF_OK;
+}
+
+// Zeroes n doubles (a kernel, not a memset, so the call stays a plain kernel
+// sequence).
+__global__ void __launch_bounds__(256) diag_clear(double* __restrict__ p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// Sums the block partials in block order (as the host does for
+// af_run_diagnostics), adds the mean flow's energy and writes the results.
+__global__ void __launch_bounds__(64) diag_finish(const double* __restrict__ part, int nblocks,
+                                                  const MeanState* __restrict__ ms, double N2,
+                                                  double* __restrict__ out, double* __restrict__ spec, int nshell) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  double se = 0, sz = 0;
+  for (int b = 0; b < nblocks; ++b) { se += part[2 * b]; sz += part[2 * b + 1]; }
+  const double emean = 0.5 * (ms->M[0] * ms->M[0] + ms->M[1] * ms->M[1]) / N2;
+  out[0] = se + emean;
+  out[1] = sz;
+  if (nshell > 0) spec[0] += emean;
+}
+
+af_status diagnostics_noalloc(const af_params& p, const Geometry& g, unsigned char* base, hipStream_t stream, int device,
+                              double* energy, double* enstrophy, double* spectrum, int nshell) {
+  if (nshell > kNoAllocMaxShells)
+    return fail(AF_EINVAL, "nshell " + std::to_string(nshell) + " exceeds the static scratch maximum of " +
+                               std::to_string(kNoAllocMaxShells) + " shells");
+  double* sp = nullptr;
+  af_status st = scratch_on(device, &sp);
+  if (st != AF_OK) return st;
+  double* part = sp;
+  double* out = sp + kScratchOut;
+  double* spec = sp + kScratchSpec;
+  const double twopi = 6.283185307179586476925286766559;
+  const double tkx = twopi / p.lx, tky = twopi / p.ly;
+  const double kunit = tkx < tky ? tkx : tky;
+  const double N = (double)p.nx * (double)p.ny;
+  const int nclear = nshell > 0 ? nshell : 1;
+  double res[2];
+  hipError_t e = hipSuccess;
+  hipLaunchKernelGGL(diag_clear, dim3((nclear + 255) / 256), dim3(256), 0, stream, spec, nclear);
+  if (g.prec == 64)
+    hipLaunchKernelGGL((diag_kernel<double>), dim3(kDiagBlocks), dim3(256), 0, stream,
+                       (const cx<double>*)(base + g.off_field[0]), g.nx2, g.ny, g.by, g.bx, tkx, tky, kunit,
+                       1.0 / (N * N), part, spec, nshell);
+  else
+    hipLaunchKernelGGL((diag_kernel<float>), dim3(kDiagBlocks), dim3(256), 0, stream,
+                       (const cx<float>*)(base + g.off_field[0]), g.nx2, g.ny, g.by, g.bx, tkx, tky, kunit,
+                       1.0 / (N * N), part, spec, nshell);
+  hipLaunchKernelGGL(diag_finish, dim3(1), dim3(64), 0, stream, (const double*)part, kDiagBlocks,
+                     (const MeanState*)(base + g.off_mean), N * N, out, spec, nshell);
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "diag launch");
+  e = hipMemcpyAsync(res, out, sizeof(res), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess && spectrum && nshell > 0)
+    e = hipMemcpyAsync(spectrum, spec, (size_t)nshell * sizeof(double), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return hip_fail(e, "diag copy");
+  if (energy) *energy = res[0];
+  if (enstrophy) *enstrophy = res[1];
+  return AF_OK;
 }
 
 }  // namespace af

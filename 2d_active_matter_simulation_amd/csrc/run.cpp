@@ -289,6 +289,7 @@ struct af_run {
   bool graph_failed = false;
   bool ready = false;
   long long steps = 0;
+  int device = 0;                   // the HIP device current at creation
 };
 
 extern "C" {
@@ -328,6 +329,12 @@ af_status af_run_create(const af_params* p, af_run** out) {
   }
   hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete r; return hip_fail(e, "stream create"); }
+  e = hipGetDevice(&r->device);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(r->stream);
+    delete r;
+    return hip_fail(e, "current device");
+  }
   e = hipMalloc(&r->base, r->g.bytes);
   if (e != hipSuccess) {
     (void)hipStreamDestroy(r->stream);
@@ -462,6 +469,12 @@ af_status af_run_diagnostics(af_run* r, double* energy, double* enstrophy, doubl
   if (!r || nshell < 0 || (nshell > 0 && !spectrum)) return fail(AF_EINVAL, "bad arguments");
   if (!r->ready) return fail(AF_EINVAL, "state not set");
   return diagnostics(r->p, r->g, r->base, r->stream, energy, enstrophy, spectrum, nshell);
+}
+
+af_status af_run_diagnostics_noalloc(af_run* r, double* energy, double* enstrophy, double* spectrum, int nshell) {
+  if (!r || nshell < 0 || (nshell > 0 && !spectrum)) return fail(AF_EINVAL, "bad arguments");
+  if (!r->ready) return fail(AF_EINVAL, "state not set");
+  return diagnostics_noalloc(r->p, r->g, r->base, r->stream, r->device, energy, enstrophy, spectrum, nshell);
 }
 
 af_status af_run_use_graph(af_run* r, int on) {

@@ -40,6 +40,8 @@ af_status load_state(const Geometry& g, unsigned char* base, const double* omega
                      hipStream_t stream);
 af_status store_state(const Geometry& g, unsigned char* base, double* omega_hat, double M[2], hipStream_t stream);
 hipError_t launch_step(const Geometry& g, const void* descs, int ninst, const void* pool, hipStream_t s);
+af_status diagnostics_noalloc(const af_params& p, const Geometry& g, unsigned char* base, hipStream_t stream, int device,
+                              double* energy, double* enstrophy, double* spectrum, int nshell);
 af_status diagnostics(const af_params& p, const Geometry& g, unsigned char* base, hipStream_t stream, double* energy,
                       double* enstrophy, double* spectrum, int nshell);
 

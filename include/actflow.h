@@ -94,6 +94,15 @@ long long af_run_steps_done(const af_run* r);
  * spectrum[s] = sum of 1/2|u^(k)|^2/N^2 over |k|/k_unit in [s-1/2, s+1/2),
  * k_unit = min(2pi/Lx, 2pi/Ly); the mean flow is in shell 0. */
 af_status af_run_diagnostics(af_run* r, double* energy, double* enstrophy, double* spectrum, int nshell);
+/* The same diagnostics without any allocation: no hipMalloc/hipFree and no
+ * host allocation.  No counterpart in the reference; for callers that cannot
+ * allocate (tight serving loops, harnesses that forbid device allocation).
+ * Its device memory is a static per-device array of 40 KiB (block partials,
+ * results and up to 4096 spectrum bins); nshell > 4096 returns AF_EINVAL.
+ * Results equal af_run_diagnostics's: energy and enstrophy bitwise, the
+ * spectrum to the rounding of its atomic sums.  Calls that overlap on one
+ * device share that array and race: serialise them. */
+af_status af_run_diagnostics_noalloc(af_run* r, double* energy, double* enstrophy, double* spectrum, int nshell);
 void af_run_destroy(af_run* r);
 
 /* Per-kernel timing with HIP events on the run's stream: runs nsteps steps,
