@@ -365,6 +365,22 @@ class Request:
     ic_kmax: float = 10.0
 
 
+def served_dt(n, lam, alpha, beta):
+    """Time step for a request of the served workload (N^2 grid, box 2pi):
+    1e-4 up to 1024^2 and 5e-5 at 2048^2, lowered further by an advective
+    CFL estimate dt <= 4.5 / (lam u_max k_max), k_max = N/3 (2/3 mask) and
+    u_max = 3 sqrt(-2 alpha / beta) (three times the saturated rms velocity,
+    sqrt(-2 alpha/beta), which the 1024^2 runs with alpha = -0.8, beta = 0.01
+    show: <E> ~ 81).  The coefficient 4.5 is the value of 1024^2 at dt = 1e-4,
+    lam = 3.5, which is stable over 300k steps; 256^2 requests at dt = 2e-4
+    with lam in [4, 5] went NaN in the served workload."""
+    base = 1e-4 if n <= 1024 else 5e-5
+    if lam <= 0 or alpha >= 0 or beta <= 0:
+        return base
+    umax = 3.0 * np.sqrt(-2.0 * alpha / beta)
+    return float(min(base, 4.5 / (lam * umax * (n / 3.0))))
+
+
 def serve(requests, hbm_budget, max_tick_steps=50, on_snapshot=None, keep_final=False):
     """Run requests through the continuous-batching loop (af_serve).
     on_snapshot(request_index, step, omega_hat, M) receives every snapshot.

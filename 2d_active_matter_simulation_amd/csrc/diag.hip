@@ -10,7 +10,7 @@
 #include <string>
 #include <vector>
 
-#include "run_impl.hpp"
+#include "engine_impl.hpp"
 
 namespace af {
 
@@ -200,6 +200,31 @@ af_status diagnostics_noalloc(const af_params& p, const Geometry& g, unsigned ch
   if (energy) *energy = res[0];
   if (enstrophy) *enstrophy = res[1];
   return AF_OK;
+}
+
+// ---- engine: mean states of a class's runs ------------------------------
+// One thread per descriptor entry copies that run's mean state (NaN flag,
+// step count, M) into out[entry], pinned host memory the device writes
+// directly, so the host reads every run's state after one event instead of
+// one copy per run.  Empty entries (nx == 0) are skipped.
+template <class T>
+__global__ void __launch_bounds__(64) gather_mean_kernel(const InstDesc<T>* __restrict__ descs, int n,
+                                                         MeanState* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const InstDesc<T>& d = descs[i];
+  if (d.nx == 0) return;
+  out[i] = *d.mean;
+}
+
+hipError_t gather_means(int prec, const void* descs, int n, MeanState* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const dim3 grid((n + 63) / 64), block(64);
+  if (prec == 64)
+    hipLaunchKernelGGL((gather_mean_kernel<double>), grid, block, 0, s, (const InstDesc<double>*)descs, n, out);
+  else
+    hipLaunchKernelGGL((gather_mean_kernel<float>), grid, block, 0, s, (const InstDesc<float>*)descs, n, out);
+  return hipGetLastError();
 }
 
 }  // namespace af

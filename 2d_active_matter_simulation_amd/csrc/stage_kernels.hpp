@@ -282,6 +282,7 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   const int inst = blockIdx.x / tiles_per_inst;
   const int tile = blockIdx.x - inst * tiles_per_inst;
   const InstDesc<T>& d = descs[inst];
+  if (d.nx == 0) return;   // an empty entry of an engine's descriptor array: the whole block exits
   // lane -> (row yy of the block, transform lane j); a wave still covers a
   // contiguous range of whole bricks
   const int t = threadIdx.x;
@@ -489,6 +490,7 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   const int inst = bid / tiles_per_inst;
   const int tile = bid - inst * tiles_per_inst;
   const InstDesc<T>& d = descs[inst];
+  if (d.nx == 0) return;   // an empty entry of an engine's descriptor array: the whole block exits
   const int t = threadIdx.x;
   const int c = t % W, j = t / W;
   const int x = tile * W + c;

@@ -263,35 +263,53 @@ def batched_leg(af, n, count, steps, seed=3):
             "step_model_frac": count * step_model_bytes(n) / (wall / steps) / 1e9 / HBM_PEAK_GBS}
 
 
-def serving_leg(af, rate, window=0.5, seed=11, horizon_scale=0.1):
-    """Config 5: Poisson arrivals at `rate`/s for `window` s; N from
-    {128..2048}^2 with weights 0.3/0.3/0.2/0.15/0.05; horizon U{1k..50k}
-    scaled by horizon_scale; snapshot every 500 steps; 200 GB budget."""
+def serving_leg(af, rate, requests=16, seed=11, horizon_scale=1.0):
+    """Config 5: Poisson arrivals at `rate`/s over a window of requests/rate s
+    (about `requests` arrivals); N from {128..2048}^2 with weights
+    0.3/0.3/0.2/0.15/0.05; horizon U{1k..50k} steps (times horizon_scale);
+    snapshot every 500 steps; alpha, beta, lambda0 as the ensemble; FP32;
+    200 GB budget; each request's dt from served_dt (1e-4 up to 1024^2, 5e-5
+    at 2048^2, lower where an advective CFL estimate asks).  The FP32 error
+    is stated for the first request of the smallest grid: FP32 vs FP64 single
+    runs of its parameters after 200 steps (the full horizons outlast the
+    flow's decorrelation, where only statistics compare)."""
     rng = np.random.default_rng(seed)
     sizes, wts = [128, 256, 512, 1024, 2048], [0.3, 0.3, 0.2, 0.15, 0.05]
+    window = requests / rate
     reqs, t = [], 0.0
     while True:
         t += rng.exponential(1.0 / rate)
         if t > window:
             break
         n = int(rng.choice(sizes, p=wts))
-        p = af.params(n, dt=2e-4 if n <= 256 else (1e-4 if n <= 1024 else 5e-5), lam=float(rng.uniform(1, 5)),
-                      alpha=float(rng.uniform(-1.2, -0.4)), beta=float(rng.uniform(0.005, 0.02)),
+        lam, alpha, beta = float(rng.uniform(1, 5)), float(rng.uniform(-1.2, -0.4)), float(rng.uniform(0.005, 0.02))
+        p = af.params(n, dt=af.served_dt(n, lam, alpha, beta), lam=lam, alpha=alpha, beta=beta,
                       gamma0=-0.045, gamma2=2.5e-5, precision=32)
-        h = int(rng.integers(1000, 50001) * horizon_scale)
+        h = max(1, int(rng.integers(1000, 50001) * horizon_scale))
         reqs.append(af.Request(arrival_s=t, params=p, horizon=h, snapshot_every=500,
                                seed=int(rng.integers(1, 1 << 30))))
     nsnap = [0]
     res, st, _ = af.serve(reqs, hbm_budget=200 << 30, max_tick_steps=50,
                           on_snapshot=lambda *a: nsnap.__setitem__(0, nsnap[0] + 1))
     failed = sum(1 for r in res if r["status"] != 0)   # e.g. a run whose state went NaN
-    return {"rate_per_s": rate, "window_s": window, "requests": len(reqs), "completed": st["completed"],
-            "failed": failed,
-            "horizon": f"U{{1k..50k}} x {horizon_scale}", "throughput": st["throughput"],
-            "step_p50_ms": st["step_p50_s"] * 1e3, "step_p99_ms": st["step_p99_s"] * 1e3,
-            "ttc_p50_s": st["ttc_p50_s"], "ttc_p99_s": st["ttc_p99_s"], "queue_p50_s": st["queue_p50_s"],
-            "queue_p99_s": st["queue_p99_s"], "max_resident": st["max_resident"], "snapshots": nsnap[0],
-            "wall_s": st["wall_s"]}
+    out = {"rate_per_s": rate, "window_s": window, "requests": len(reqs), "completed": st["completed"],
+           "failed": failed, "horizon": f"U{{1k..50k}} x {horizon_scale}", "throughput": st["throughput"],
+           "step_p50_ms": st["step_p50_s"] * 1e3, "step_p99_ms": st["step_p99_s"] * 1e3,
+           "ttc_p50_s": st["ttc_p50_s"], "ttc_p99_s": st["ttc_p99_s"], "queue_p50_s": st["queue_p50_s"],
+           "queue_p99_s": st["queue_p99_s"], "max_resident": st["max_resident"], "snapshots": nsnap[0],
+           "chunks": st["ticks"], "wall_s": st["wall_s"]}
+    if reqs:
+        i = min(range(len(reqs)), key=lambda j: (reqs[j].params.nx, j))
+        r = reqs[i]
+        ic = af.ic_bandlimited(r.params.nx, seed=r.seed, amp=r.ic_amp, kmax=r.ic_kmax)
+        run = af.Run(r.params)
+        run.set_state(ic)
+        k = min(200, r.horizon)
+        run.step(k)
+        err = fp32_error(af, r.params, ic, (0.0, 0.0), k, run.get_state()[0])
+        run.close()
+        out["fp32_error"] = {"request": i, "grid": [r.params.nx, r.params.ny], **err}
+    return out
 
 
 def cpu_configs():
@@ -429,18 +447,16 @@ def main():
         return
     if args.serving_requests > 0:
         for r in (10, 50, 200):
-            print(json.dumps({"serving": serving_leg(af, r, window=args.serving_requests / r, horizon_scale=1.0)}),
+            print(json.dumps({"serving": serving_leg(af, r, requests=args.serving_requests)}),
                   flush=True)
         return
     if args.batch > 0:
         print(json.dumps({"batched": batched_leg(af, N, args.batch, args.steps)}))
         return
-    head = run_workload(af, N, 32, args.steps, args.warmup, host_io=True)
-    name, alg, kms, achieved = roofline(af, head)
-    traffic = read_traffic(f"{N}_fp32")
-    tr_bytes = None
-    if traffic and f"{name}_bytes_per_launch" in traffic:
-        tr_bytes = traffic[f"{name}_bytes_per_launch"]
+    # The other single-run workloads go first: the headline's W warm-up steps
+    # then follow other GPU work, as in a process that has been stepping, instead
+    # of being the first GPU work of the process (its first ~2 ms run ~8 %
+    # slower while the device comes out of idle).
     extra = {}
     if not args.no_extra:
         for n, prec, steps in ((1024, 64, max(200, args.steps // 4)), (2048, 32, max(100, args.steps // 8)),
@@ -454,8 +470,14 @@ def main():
                                       / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
             if r["fp32_error"]:
                 extra[f"{n}_fp{prec}"]["fp32_error"] = r["fp32_error"]
-        if N == 1024:
-            extra["fp32_over_fp64_1024"] = head["value"] / extra["1024_fp64"]["value"]
+    head = run_workload(af, N, 32, args.steps, args.warmup, host_io=True)
+    name, alg, kms, achieved = roofline(af, head)
+    traffic = read_traffic(f"{N}_fp32")
+    tr_bytes = None
+    if traffic and f"{name}_bytes_per_launch" in traffic:
+        tr_bytes = traffic[f"{name}_bytes_per_launch"]
+    if not args.no_extra and N == 1024:
+        extra["fp32_over_fp64_1024"] = head["value"] / extra["1024_fp64"]["value"]
     out = {
         "metric": METRIC,
         "value": head["value"],
