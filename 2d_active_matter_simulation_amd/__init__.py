@@ -132,6 +132,8 @@ def lib() -> ctypes.CDLL:
         L.af_engine_remove.argtypes = [vp, ctypes.c_int]
         L.af_engine_step.argtypes = [vp, ctypes.c_int]
         L.af_engine_step_slots.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int]
+        L.af_engine_set_single_launch.argtypes = [vp, ctypes.c_int]
+        L.af_engine_single_launch_active.argtypes = [vp]
         L.af_engine_get_state.argtypes = [vp, ctypes.c_int, dp, dp]
         L.af_engine_get_energy.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         L.af_engine_steps_done.argtypes = [vp, ctypes.c_int]
@@ -320,6 +322,15 @@ class Engine:
         else:
             arr = (ctypes.c_int * len(slots))(*slots)
             _check(lib().af_engine_step_slots(self._h, arr, len(slots), nsteps))
+
+    def set_single_launch(self, on: bool):
+        """One launch per pass over all size classes when they form an
+        instantiated mix (af_engine_set_single_launch)."""
+        _check(lib().af_engine_set_single_launch(self._h, 1 if on else 0))
+
+    @property
+    def single_launch_active(self) -> bool:
+        return bool(lib().af_engine_single_launch_active(self._h))
 
     def get_state(self, slot: int):
         p = self.params[slot]

@@ -191,17 +191,10 @@ static bool capture_level(Class& c, int level) {
   return true;
 }
 
-af_status engine_enqueue(af_engine* e, Class& c, int k) {
+// Compaction and a mean-state mirror for every entry, before a launch.
+static af_status class_prepare(af_engine* e, Class& c) {
   af_status st = compact(e, c);
   if (st != AF_OK) return st;
-  if (c.launched == 0 || k <= 0) return AF_OK;
-  if (c.graph_launched != c.launched) {
-    // the previous graphs may still be running on the class's stream
-    hipError_t he = hipStreamSynchronize(c.stream);
-    if (he != hipSuccess) return hip_fail(he, "class sync");
-    drop_graphs(c);
-    c.graph_launched = c.launched;
-  }
   if (c.mirror_cap < c.launched) {
     hipError_t he = hipStreamSynchronize(c.stream);
     if (he == hipSuccess && c.mirror) he = hipHostFree(c.mirror);
@@ -210,6 +203,20 @@ af_status engine_enqueue(af_engine* e, Class& c, int k) {
     if (he == hipSuccess) he = hipHostMalloc((void**)&c.mirror, (size_t)c.cap * sizeof(MeanState), hipHostMallocCoherent);
     if (he != hipSuccess) { c.mirror = nullptr; return fail(AF_ENOMEM, "mean-state mirror"); }
     c.mirror_cap = c.cap;
+  }
+  return AF_OK;
+}
+
+af_status engine_enqueue(af_engine* e, Class& c, int k) {
+  af_status st = class_prepare(e, c);
+  if (st != AF_OK) return st;
+  if (c.launched == 0 || k <= 0) return AF_OK;
+  if (c.graph_launched != c.launched) {
+    // the previous graphs may still be running on the class's stream
+    hipError_t he = hipStreamSynchronize(c.stream);
+    if (he != hipSuccess) return hip_fail(he, "class sync");
+    drop_graphs(c);
+    c.graph_launched = c.launched;
   }
   auto enqueue_level = [&](int level) -> hipError_t {
     if (c.gexec[level] || capture_level(c, level)) return hipGraphLaunch(c.gexec[level], c.stream);
@@ -226,6 +233,122 @@ af_status engine_enqueue(af_engine* e, Class& c, int k) {
     if (left & (1 << l)) he = enqueue_level(l);
   if (he == hipSuccess) he = gather_means(c.g.prec, c.descs, c.launched, c.mirror, c.stream);
   if (he != hipSuccess) return hip_fail(he, "engine step");
+  return AF_OK;
+}
+
+// ---- single-launch mode ----------------------------------------------------
+
+static void drop_mix_graphs(af_engine* e) {
+  for (int l = 0; l < kEngGraphLevels; ++l) {
+    if (e->mix_exec[l]) (void)hipGraphExecDestroy(e->mix_exec[l]);
+    if (e->mix_graph[l]) (void)hipGraphDestroy(e->mix_graph[l]);
+    e->mix_exec[l] = nullptr;
+    e->mix_graph[l] = nullptr;
+  }
+  e->mix = nullptr;
+  e->mix_key.clear();
+}
+
+static bool same_schedule(const Schedule& a, const Schedule& b) {
+  return a.nx2 == b.nx2 && a.ny == b.ny && a.by == b.by && a.bx == b.bx && a.row_r == b.row_r &&
+         a.row_g == b.row_g && a.col_r == b.col_r && a.col_gf == b.col_gf && a.col_gi == b.col_gi &&
+         a.col_w == b.col_w && a.pp == b.pp;
+}
+
+// The instantiated mix made of exactly the classes `keys` (its classes in
+// launch order into `order`), or null.
+static const MultiCfg* find_mix(af_engine* e, const std::vector<ClassKey>& keys, std::vector<Class*>& order) {
+  int n = 0;
+  const MultiCfg* m = multi_cfgs(&n);
+  for (int i = 0; i < n; ++i) {
+    const MultiCfg& cfg = m[i];
+    if (cfg.nclass != (int)keys.size()) continue;
+    std::vector<Class*> ord;
+    for (int k = 0; k < cfg.nclass; ++k) {
+      Class* hit = nullptr;
+      for (const ClassKey& key : keys) {
+        const Schedule* s = std::get<3>(key);
+        if (std::get<0>(key) == cfg.precision && s && same_schedule(*s, cfg.cls[k])) {
+          hit = &e->classes.at(key);
+          break;
+        }
+      }
+      if (!hit) break;
+      ord.push_back(hit);
+    }
+    if ((int)ord.size() == cfg.nclass) {
+      order = ord;
+      return &cfg;
+    }
+  }
+  return nullptr;
+}
+
+// k steps of every run of the mix's classes, each pass one mixed launch on
+// the engine's stream, then the classes' mean-state gathers.  Asynchronous.
+static af_status mix_enqueue(af_engine* e, const MultiCfg* mix, const std::vector<Class*>& cls, int k) {
+  for (Class* c : cls) {
+    af_status st = class_prepare(e, *c);
+    if (st != AF_OK) return st;
+  }
+  hipError_t he = hipSuccess;
+  for (Class* c : cls)   // descriptor edits on the class streams land before the mixed launches
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+  if (he != hipSuccess) return hip_fail(he, "class sync");
+  MultiArgs ra{}, ca{};
+  std::vector<const void*> key;
+  for (size_t i = 0; i < cls.size(); ++i) {
+    const Class& c = *cls[i];
+    ra.descs[i] = ca.descs[i] = c.descs;
+    ra.pool[i] = ca.pool[i] = c.pool;
+    ra.tiles[i] = c.g.row_tiles;
+    ca.tiles[i] = c.g.col_tiles;
+    ra.nblocks[i] = c.launched * c.g.row_tiles;
+    ca.nblocks[i] = c.launched * c.g.col_tiles;
+    key.push_back(c.descs);
+    key.push_back(reinterpret_cast<const void*>((size_t)c.launched));
+  }
+  if (e->mix != mix || e->mix_key != key) {
+    drop_mix_graphs(e);
+    e->mix = mix;
+    e->mix_key = key;
+  }
+  auto steps = [&](int n) -> hipError_t {
+    hipError_t r = hipSuccess;
+    for (int i = 0; i < n && r == hipSuccess; ++i)
+      for (int stage = 1; stage <= 4 && r == hipSuccess; ++stage) {
+        r = mix->row(ra, e->stream);
+        if (r == hipSuccess) r = mix->col(ca, stage, e->stream);
+      }
+    return r;
+  };
+  auto level = [&](int l) -> hipError_t {
+    if (!e->mix_exec[l]) {
+      hipError_t r = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+      if (r == hipSuccess) {
+        const hipError_t lr = steps(1 << l);
+        r = hipStreamEndCapture(e->stream, &e->mix_graph[l]);
+        if (lr == hipSuccess && r == hipSuccess) r = hipGraphInstantiate(&e->mix_exec[l], e->mix_graph[l], nullptr, nullptr, 0);
+        else if (r == hipSuccess) r = lr;
+      }
+      if (r != hipSuccess || !e->mix_exec[l]) {   // capture failed: direct launches
+        (void)hipGetLastError();
+        if (e->mix_graph[l]) (void)hipGraphDestroy(e->mix_graph[l]);
+        e->mix_graph[l] = nullptr;
+        e->mix_exec[l] = nullptr;
+        return steps(1 << l);
+      }
+    }
+    return hipGraphLaunch(e->mix_exec[l], e->stream);
+  };
+  const int top = kEngGraphLevels - 1;
+  int left = k;
+  while (left >= (1 << top) && he == hipSuccess) { he = level(top); left -= 1 << top; }
+  for (int l = top - 1; l >= 0 && he == hipSuccess; --l)
+    if (left & (1 << l)) he = level(l);
+  for (Class* c : cls)
+    if (he == hipSuccess) he = gather_means(c->g.prec, c->descs, c->launched, c->mirror, e->stream);
+  if (he != hipSuccess) return hip_fail(he, "mixed step");
   return AF_OK;
 }
 
@@ -342,14 +465,26 @@ af_status af_engine_step_slots(af_engine* e, const int* slots, int nslots, int n
     idle.emplace_back(&c, out);
   }
   af_status st = he == hipSuccess ? AF_OK : hip_fail(he, "sit-out descriptors");
+  std::vector<Class*> mix_order;
+  const MultiCfg* mix = nullptr;
+  if (e->single_launch) {
+    std::vector<ClassKey> keys;
+    for (auto& kv : by) keys.push_back(kv.first);
+    mix = find_mix(e, keys, mix_order);
+  }
   int done = 0;
   while (st == AF_OK && done < nsteps) {
     const int k = std::min(nsteps - done, kRingCap);
-    for (auto& kv : by)
-      if (st == AF_OK) st = engine_enqueue(e, e->classes.at(kv.first), k);
-    for (auto& kv : by)
-      if (st == AF_OK && (he = hipStreamSynchronize(e->classes.at(kv.first).stream)) != hipSuccess)
-        st = hip_fail(he, "engine sync");
+    if (mix) {
+      st = mix_enqueue(e, mix, mix_order, k);
+      if (st == AF_OK && (he = hipStreamSynchronize(e->stream)) != hipSuccess) st = hip_fail(he, "engine sync");
+    } else {
+      for (auto& kv : by)
+        if (st == AF_OK) st = engine_enqueue(e, e->classes.at(kv.first), k);
+      for (auto& kv : by)
+        if (st == AF_OK && (he = hipStreamSynchronize(e->classes.at(kv.first).stream)) != hipSuccess)
+          st = hip_fail(he, "engine sync");
+    }
     if (st == AF_OK)
       for (int id : list) e->slots[id].steps += k;
     done += k;
@@ -373,6 +508,21 @@ af_status af_engine_step_slots(af_engine* e, const int* slots, int nslots, int n
                                std::to_string(ms.nan_step) + " timesteps");
   }
   return AF_OK;
+}
+
+af_status af_engine_set_single_launch(af_engine* e, int on) {
+  if (!e) return fail(AF_EINVAL, "engine is NULL");
+  e->single_launch = on != 0;
+  return AF_OK;
+}
+
+int af_engine_single_launch_active(af_engine* e) {
+  if (!e || !e->single_launch) return 0;
+  std::vector<ClassKey> keys;
+  for (const auto& kv : e->classes)
+    if (kv.second.members > 0) keys.push_back(kv.first);
+  std::vector<Class*> order;
+  return find_mix(e, keys, order) ? 1 : 0;
 }
 
 af_status af_engine_step(af_engine* e, int nsteps) {
@@ -431,6 +581,7 @@ void af_engine_destroy(af_engine* e) {
   if (!e) return;
   (void)hipStreamSynchronize(e->stream);
   for (auto& kv : e->classes) (void)hipStreamSynchronize(kv.second.stream);
+  drop_mix_graphs(e);
   for (auto& kv : e->classes) {
     Class& c = kv.second;
     drop_graphs(c);

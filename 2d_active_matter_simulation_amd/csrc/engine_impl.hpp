@@ -65,6 +65,14 @@ struct af_engine {
   size_t pooled = 0;                    // bytes in free_blocks
   std::vector<af::PendingBlock> pending;
   std::vector<void*> graveyard;         // replaced descriptor arrays, freed at destroy
+  // single-launch mode: when the classes stepped together form an
+  // instantiated mix (af::multi_cfgs), each pass is one mixed launch over all
+  // of them on `stream` instead of one launch per class on the class streams
+  bool single_launch = false;
+  const af::MultiCfg* mix = nullptr;     // the mix of the captured graphs
+  std::vector<const void*> mix_key;      // descriptor arrays and entry counts they cover
+  hipGraph_t mix_graph[af::kEngGraphLevels] = {};
+  hipGraphExec_t mix_exec[af::kEngGraphLevels] = {};
 };
 
 namespace af {

@@ -53,6 +53,15 @@ STORE_VARIANT_BASES = {
 }
 
 
+# Mixes instantiated as one launch per pass (the ensemble's single-launch
+# mode): per precision, lists of (n, schedule) for n x n classes, in launch
+# order (largest first).  Every schedule of a mix has 256-lane blocks in both
+# passes: one launch has one block size.
+MULTI = {
+    32: [[(1024, (4, 2, 8, 1, 8, 1, 1, 2, 0)), (512, (4, 4, 4, 1, 8, 1, 1, 4, 0)), (256, (8, 4, 4, 1, 4, 1, 1, 4, 0))]],
+}
+
+
 def default_schedule(prec, nx, ny):
     """(by, bx, row_r, row_g, col_r, col_gf, col_gi, col_w, pp)"""
     nx2 = nx // 2
@@ -105,6 +114,8 @@ def candidates(prec, n):
         for rg, gf, gi in ((1, 2, 3), (3, 2, 1), (3, 1, 1), (1, 2, 1), (1, 1, 1)):
             for o in (1, 3, 7, 11, 15):
                 alts.append((4, 2, 4, rg, 4, gf, gi, 2 if n == 1024 else 1, o))
+    for mix in MULTI.get(prec, []):   # the schedules of the mixed launches
+        alts += [s for m, s in mix if m == n]
     if prec == 32:
         for b in STORE_VARIANT_BASES.get(n, []):
             opts = list(range(2, 16, 2)) + ([16 + o for o in range(0, 16, 2)] if b[4] == 16 else [])
@@ -226,9 +237,53 @@ namespace af {{
 """
 
 
+MULTI_FILE = """// Mixed launches: one launch per pass over the size classes of each mix.
+// Generated by gen_kernel_tables.py -- edit that script, not this file.
+#include "multi_pass.hpp"
+
+namespace af {{
+namespace {{
+const MultiCfg kMulti[] = {{
+{entries}
+}};
+}}  // namespace
+
+const MultiCfg* multi_cfgs(int* n) {{
+  *n = {count};
+  return {ptr};
+}}
+
+}}  // namespace af
+"""
+
+
+def gen_multi(outdir):
+    entries = []
+    for prec, mixes in MULTI.items():
+        T = "float" if prec == 32 else "double"
+        for mix in mixes:
+            assert len(mix) <= 4
+            cls, rows, cols = [], [], []
+            for n, s in mix:
+                by, bx, rr, rg, cr, gf, gi, cw, pp = s
+                assert feasible(prec, n, n, s) and s in candidates(prec, n), (n, s)
+                assert by * (n // 2 // rr) == 256 and cw * (n // cr) == 256, (n, s)
+                cls.append("{" + ", ".join(map(str, (n // 2, n, by, bx, rr, rg, cr, gf, gi, cw, pp, 0))) + "}")
+                rows.append(f"RowPass<{T}, {n // 2}, {by}, {rr}, {bx}, {rg}, {pp & ROW_OPTS}>")
+                cols.append(f"ColCfgT<{T}, {n}, {cw}, {cr}, {by}, {bx}, {gf}, {gi}, {pp & COL_OPTS}>")
+            entries.append(f"    {{{prec}, {len(mix)}, {{{', '.join(cls)}}},\n     launch_row_multi<{', '.join(rows)}>,\n"
+                           f"     launch_col_multi<{', '.join(cols)}>}},")
+    text = MULTI_FILE.format(entries="\n".join(entries) if entries else "    {}",
+                             count=len(entries), ptr="kMulti" if entries else "nullptr")
+    with open(os.path.join(outdir, "kernels_multi.hip"), "w") as f:
+        f.write(text)
+    return len(entries)
+
+
 if __name__ == "__main__":
     if len(sys.argv) != 2:
         sys.exit("usage: gen_kernel_tables.py OUTDIR")
     os.makedirs(sys.argv[1], exist_ok=True)
     for prec in (32, 64):
         print(prec, gen(prec, sys.argv[1]))
+    print("mixes", gen_multi(sys.argv[1]))

@@ -48,6 +48,28 @@ const Schedule* active_schedule(int precision, int nx2, int ny, bool batched = f
 bool select_schedule(int precision, int nx2, int ny, int index);
 const RowCfg* find_row_cfg(int precision, const Schedule* s);
 const ColCfg* find_col_cfg(int precision, const Schedule* s);
+// Mixed launches (one launch per pass over several size classes, the
+// ensemble's single-launch mode; multi_pass.hpp).  Arguments: per class, in
+// launch order, its descriptor array, twiddle pool, tiles per instance and
+// blocks in the launch (entries x tiles).
+constexpr int kMaxMulti = 4;
+struct MultiArgs {
+  const void* descs[kMaxMulti];
+  const void* pool[kMaxMulti];
+  int tiles[kMaxMulti];
+  int nblocks[kMaxMulti];
+};
+using MultiRowLaunch = hipError_t (*)(const MultiArgs& a, hipStream_t s);
+using MultiColLaunch = hipError_t (*)(const MultiArgs& a, int stage, hipStream_t s);
+// An instantiated mix: its classes' schedules in launch order (the row
+// arguments use the row tiles, the column arguments the column tiles).
+struct MultiCfg {
+  int precision, nclass;
+  Schedule cls[kMaxMulti];
+  MultiRowLaunch row;
+  MultiColLaunch col;
+};
+const MultiCfg* multi_cfgs(int* n);
 // When set (profiling), the next row/column launch on this thread records
 // these events at the kernel's own start and end (hipExtLaunchKernel).
 struct LaunchEvents {

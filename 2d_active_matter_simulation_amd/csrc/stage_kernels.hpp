@@ -264,23 +264,49 @@ __device__ __forceinline__ double lds_wave_sum(const double* red, int k) {
 // exchange (more LDS); 4 = streaming stores of P, Q (otherwise device-scope
 // write-through stores).  U, V, W are read with streaming loads.
 // ---------------------------------------------------------------------------
+// RowPass<...>::run is the body of one block (tile `block` of the launch,
+// block size NT, LDS `sm`); row_pass is the kernel of one size class, and a
+// mixed launch (multi_pass.hpp) calls the bodies of several classes.
+template <class T, int NX2, int RB, int R, int BX, int G, int PP>
+struct RowPass {
+  using Real = T;
+  static constexpr int TR = NX2 / R;
+  static constexpr int NT = RB * TR;
+  using View = LdsView<T, NX2, RB, R, G, PP & 1>;
+  struct Smem {
+    cx<T> lds_mem[View::kElems];
+    cx<T> tw_h[TwLayout<NX2, R>::kAlloc];   // the half-length transforms
+    cx<T> tw_n[NX2];                        // exp(-2 pi i k / Nx), k < Nx/2, the even/odd split
+    double red[4 * ((NT + 63) / 64)];
+  };
+  __device__ static void run(const InstDesc<T>* __restrict__ descs, int tiles_per_inst,
+                             const cx<T>* __restrict__ pool, int block, Smem& sm);
+};
+
 template <class T, int NX2, int RB, int R, int BX, int G, int PP>
 __global__ void __launch_bounds__(RB * (NX2 / R))
 row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>* __restrict__ pool) {
-  constexpr int TR = NX2 / R;
-  constexpr int NT = RB * TR;
+  using P = RowPass<T, NX2, RB, R, BX, G, PP>;
+  __shared__ typename P::Smem sm;
+  P::run(descs, tiles_per_inst, pool, blockIdx.x, sm);
+}
+
+template <class T, int NX2, int RB, int R, int BX, int G, int PP>
+__device__ __forceinline__ void RowPass<T, NX2, RB, R, BX, G, PP>::run(const InstDesc<T>* __restrict__ descs,
+                                                                      int tiles_per_inst,
+                                                                      const cx<T>* __restrict__ pool, int block,
+                                                                      Smem& sm) {
   static_assert(NX2 % BX == 0 && TR % BX == 0, "row lanes must cover whole bricks");
   static_assert(G == 1 || G == 3, "group of 1 or 3 fields");
   constexpr int GR = G == 3 ? 2 : 1;   // R2C group
-  using View = LdsView<T, NX2, RB, R, G, PP & 1>;
   constexpr int kCP = (PP & 4) ? kCacheNt : kCacheSc1;
-  __shared__ cx<T> lds_mem[View::kElems];
-  __shared__ cx<T> tw_h[TwLayout<NX2, R>::kAlloc];   // the half-length transforms
-  __shared__ cx<T> tw_n[NX2];          // exp(-2 pi i k / Nx), k < Nx/2, the even/odd split
-  __shared__ double red[4 * ((NT + 63) / 64)];
+  cx<T>* const lds_mem = sm.lds_mem;
+  cx<T>* const tw_h = sm.tw_h;
+  cx<T>* const tw_n = sm.tw_n;
+  double* const red = sm.red;
 
-  const int inst = blockIdx.x / tiles_per_inst;
-  const int tile = blockIdx.x - inst * tiles_per_inst;
+  const int inst = block / tiles_per_inst;
+  const int tile = block - inst * tiles_per_inst;
   const InstDesc<T>& d = descs[inst];
   if (d.nx == 0) return;   // an empty entry of an engine's descriptor array: the whole block exits
   // lane -> (row yy of the block, transform lane j); a wave still covers a
@@ -368,9 +394,13 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
     }
   }
 
-  // Real-space products (nonl_subfunc_1) and the two combined fields.
+  // Real-space products (nonl_subfunc_1) and the two combined fields.  The
+  // lane's 2R points are summed in T (sum |u|^2 reuses Q = |u|^2 of the cubic
+  // term), then converted once: the block sum below and the sum over row
+  // blocks (column pass, tile 0) are FP64 in a fixed order.  In FP32 a lane
+  // sum of 16-32 FP32 products is as accurate as the products themselves.
   const T lam = (T)d.lam, bet = (T)d.beta;
-  double esum = 0, gxs = 0, gys = 0;
+  T esum_l = 0, gxs_l = 0, gys_l = 0;
   cx<T> f2[2][R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -385,13 +415,14 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
       const T Cx = Q * ux, Cy = Q * uy;
       o1[h] = lam * A + bet * Cy;
       o2[h] = lam * Wp - bet * Cx;
-      esum += (double)ux * (double)ux + (double)uy * (double)uy;
-      gxs += (double)Cx;
-      gys += (double)Cy;
+      esum_l += Q;
+      gxs_l += Cx;
+      gys_l += Cy;
     }
     f2[0][r] = cx<T>{o1[0], o1[1]};
     f2[1][r] = cx<T>{o2[0], o2[1]};
   }
+  const double esum = (double)esum_l, gxs = (double)gxs_l, gys = (double)gys_l;
 
   // Row R2C of packed real pairs: Z = FFT_M(z), then
   // 2 X[k] = (Z[k] + conj Z[M-k]) - i e^{-2 pi i k/N} (Z[k] - conj Z[M-k]).
@@ -462,14 +493,39 @@ row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
 // 8 = streaming loads of P, Q; 16 = FP32 integrating factors from the table
 // at radix 16 too.
 // ---------------------------------------------------------------------------
+// ColPass<...>::run is the body of one block (block `block` of `nblocks` of
+// the class in the launch); col_pass is the kernel of one size class.
+template <class T, int NY, int W, int R, int BY, int BX, int GF, int GI, int PP, int stage>
+struct ColPass {
+  using Real = T;
+  static constexpr int TC = NY / R;
+  static constexpr int NT = W * TC;
+  static constexpr int GMAX = GF > GI ? GF : GI;
+  using View = LdsView<T, NY, W, R, GMAX, PP & 1>;
+  struct Smem {
+    cx<T> lds_mem[View::kElems];
+    cx<T> tw_y[TwLayout<NY, R>::kAlloc];
+    double mean_in[4];                       // tile 0: M, M_old
+    double mean_red[3 * ((NT + 63) / 64)];   // tile 0: wave sums of the row partials
+    long long nstep_in;
+  };
+  __device__ static void run(const InstDesc<T>* __restrict__ descs, int tiles_per_inst,
+                             const cx<T>* __restrict__ pool, int block, int nblocks, Smem& sm);
+};
+
 template <class T, int NY, int W, int R, int BY, int BX, int GF, int GI, int PP, int stage>
 __global__ void __launch_bounds__(W * (NY / R))
 col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>* __restrict__ pool) {
-  constexpr int TC = NY / R;
-  constexpr int NT = W * TC;
-  constexpr int GMAX = GF > GI ? GF : GI;
+  using P = ColPass<T, NY, W, R, BY, BX, GF, GI, PP, stage>;
+  __shared__ typename P::Smem sm;
+  P::run(descs, tiles_per_inst, pool, blockIdx.x, gridDim.x, sm);
+}
+
+template <class T, int NY, int W, int R, int BY, int BX, int GF, int GI, int PP, int stage>
+__device__ __forceinline__ void ColPass<T, NY, W, R, BY, BX, GF, GI, PP, stage>::run(
+    const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>* __restrict__ pool, int block,
+    int nblocks, Smem& sm) {
   static_assert((GF == 1 || GF == 2) && (GI == 1 || GI == 3), "field groups");
-  using View = LdsView<T, NY, W, R, GMAX, PP & 1>;
   constexpr int kCP = (PP & 2) ? kCacheNt : kCacheDefault;     // field stores
   constexpr int kPqCP = (PP & 8) ? kCacheNt : kCacheDefault;   // P, Q loads
   static_assert(NY > R, "every column transform has a barrier before its first LDS or twiddle read");
@@ -477,16 +533,16 @@ col_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>*
   // set 1 needs no barrier before it: its first stores go to set 0.  Tile 0
   // keeps the barrier after the projection, whose exchange reads set 0.
   constexpr bool kSkipPreFft = View::kBuf != 0 && FftMulti<NY, R, -1, 1, W, T, View::kBuf>::kLastLoadSet == 1;
-  __shared__ cx<T> lds_mem[View::kElems];
-  __shared__ cx<T> tw_y[TwLayout<NY, R>::kAlloc];
-  __shared__ double mean_in[4];   // tile 0: M, M_old
-  __shared__ double mean_red[3 * ((NT + 63) / 64)];   // tile 0: wave sums of the row partials
-  __shared__ long long nstep_in;
+  cx<T>* const lds_mem = sm.lds_mem;
+  cx<T>* const tw_y = sm.tw_y;
+  double* const mean_in = sm.mean_in;
+  double* const mean_red = sm.mean_red;
+  long long& nstep_in = sm.nstep_in;
 
   // a tile reads BY x W elements of each BY-row band: how many column tiles
   // share a 128-byte line
   constexpr int kLine = 128 / (BY * W * (int)sizeof(cx<T>));
-  const int bid = xcd_tile_order<kLine>(blockIdx.x, gridDim.x);
+  const int bid = xcd_tile_order<kLine>(block, nblocks);
   const int inst = bid / tiles_per_inst;
   const int tile = bid - inst * tiles_per_inst;
   const InstDesc<T>& d = descs[inst];

@@ -49,6 +49,10 @@ def step_model_bytes(n, prec=32):
 #                (+ the FP32 integrating-factor table, see col_fields)
 ROW_FIELDS = 5.0
 COL_FIELDS = (7 + 8 + 8 + 7) / 4.0
+# The ensemble mix instantiated for single-launch mode: per grid, the schedule
+# (by, bx, row radix, row group, col radix, col fwd group, col inv group,
+# col width, options) with 256-lane blocks in both passes.
+MIX_SCHEDULES = {1024: (4, 2, 8, 1, 8, 1, 1, 2, 0), 512: (4, 4, 4, 1, 8, 1, 1, 4, 0), 256: (8, 4, 4, 1, 4, 1, 1, 4, 0)}
 
 
 def col_fields(af, n, prec):
@@ -190,6 +194,29 @@ def ensemble_leg(af, steps=200, seed=7, measured_sequential=False):
         # the engine's batched schedules differ from the single-run ones, so the
         # two agree to FP32 rounding grown over the steps, not bitwise
         out["max_rel_diff_engine_vs_alone"] = worst
+    # The same 64 runs with the 256-lane schedules of the instantiated mix:
+    # per-class launches (concurrent streams) against single-launch mode (one
+    # launch per pass over all three classes)
+    modes = {}
+    try:
+        for n, sc in MIX_SCHEDULES.items():
+            af.select_schedule(32, n, n, af.schedules(32, n).index(sc))
+        for single in (False, True):
+            e2 = af.Engine(200 << 30)
+            e2.set_single_launch(single)
+            for _, p, ic in runs:
+                e2.add(p, ic)
+            e2.step(10)
+            t0 = time.perf_counter()
+            e2.step(steps)
+            w2 = time.perf_counter() - t0
+            modes["single_launch" if single else "per_class_launches"] = {
+                "value": pts * steps / w2, "ms_per_step": w2 / steps * 1e3, "single_launch_active": e2.single_launch_active}
+            e2.close()
+    finally:
+        for n in MIX_SCHEDULES:
+            af.select_schedule(32, n, n, -1)
+    out["mix_schedules"] = {"schedules": {f"{n}^2": list(sc) for n, sc in MIX_SCHEDULES.items()}, **modes}
     # FP32 error of the first instance of each size class against FP64
     first = {}
     for slot, p, ic in runs:

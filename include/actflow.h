@@ -162,6 +162,16 @@ af_status af_engine_remove(af_engine* e, int slot);
 af_status af_engine_step(af_engine* e, int nsteps);
 /* Advance only the listed slots (same batching within the list). */
 af_status af_engine_step_slots(af_engine* e, const int* slots, int nslots, int nsteps);
+/* Single-launch mode (off by default): when the size classes stepped
+ * together are exactly an instantiated mix (the FP32 256^2 + 512^2 + 1024^2
+ * ensemble with 256-lane schedules in both passes), every pass of a step is
+ * ONE launch over all runs of all those classes instead of one launch per
+ * class on concurrent streams; results are bitwise those of the per-class
+ * launches with the same schedules.  Otherwise stepping is unchanged.  No
+ * reference counterpart (the reference runs one grid per process). */
+af_status af_engine_set_single_launch(af_engine* e, int on);
+/* 1 if single-launch mode is on and the engine's current classes form a mix. */
+int af_engine_single_launch_active(af_engine* e);
 af_status af_engine_get_state(af_engine* e, int slot, double* omega_hat, double M[2]);
 /* Copies the last n per-step energies of a slot (oldest first). */
 af_status af_engine_get_energy(af_engine* e, int slot, int n, double* energy);

@@ -6,6 +6,8 @@ import os
 import numpy as np
 import pytest
 
+from conftest import rel_l2
+
 pytestmark = pytest.mark.gpu
 
 
@@ -117,7 +119,7 @@ def test_engine_schedule_matches_single_run_to_rounding(af):
     one (radix, fields per exchange); both integrate the same equation, so
     the states agree to FP32 rounding after a few steps (relative L2 with
     the half-spectrum weights, tolerance 1e-5)."""
-    from conftest import rel_l2
+
     n = 1024
     p = af.params(n, dt=1e-4, precision=32, lam=3.5, alpha=-0.8, beta=0.01, gamma0=-0.045, gamma2=2.5e-5)
     w0 = af.ic_bandlimited(n, seed=5, amp=1.0, kmax=10.0)
@@ -158,3 +160,62 @@ def test_engine_batched_schedule_2048_against_oracle(af, orc):
         assert err <= 2e-6, err
         assert np.allclose(m, mo, rtol=2e-5)
     eng.close()
+
+
+# The mix instantiated for single-launch mode: 256-lane blocks in both passes.
+MIX = {1024: (4, 2, 8, 1, 8, 1, 1, 2, 0), 512: (4, 4, 4, 1, 8, 1, 1, 4, 0), 256: (8, 4, 4, 1, 4, 1, 1, 4, 0)}
+
+
+def test_single_launch_mix_bitwise_and_oracle(af, orc):
+    """Single-launch mode (one launch per pass over the 256^2, 512^2 and 1024^2
+    classes) against per-class launches with the same schedules: bitwise equal
+    states and energies after 13 steps, with an empty descriptor entry in a
+    class (a removed run); after one step each size's first run is within
+    2e-6 of the FP64 oracle."""
+    rng = np.random.default_rng(11)
+    specs = []
+    for n, cnt in ((256, 5), (512, 3), (1024, 2)):
+        for i in range(cnt):
+            p = af.params(n, dt=2e-4 if n <= 256 else 1e-4, lam=float(rng.uniform(1, 5)),
+                          alpha=float(rng.uniform(-1.2, -0.4)), beta=float(rng.uniform(0.005, 0.02)),
+                          gamma0=-0.045, gamma2=2.5e-5, precision=32)
+            specs.append((n, p, af.ic_bandlimited(n, seed=int(rng.integers(1, 1 << 30)), amp=0.5, kmax=10.0)))
+    try:
+        for n, sc in MIX.items():
+            af.select_schedule(32, n, n, af.schedules(32, n).index(sc))
+        engines = []
+        for single in (True, False):
+            eng = af.Engine(8 << 30)
+            eng.set_single_launch(single)
+            slots = [eng.add(p, w) for _, p, w in specs]
+            eng.remove(slots[1])            # a hole in the 256^2 class
+            engines.append((eng, slots))
+        (es, ss), (ef, sf) = engines
+        assert es.single_launch_active and not ef.single_launch_active
+        for eng, _ in engines:
+            eng.step(1)
+        firsts = {}
+        for k, (n, p, w0) in enumerate(specs):
+            firsts.setdefault(n, k)
+        for n, k in firsts.items():
+            q = af.AfParams.from_buffer_copy(specs[k][1])
+            q.precision = 64
+            o = orc.Oracle.from_params(q, nthreads=16)
+            o.set_state(specs[k][2])
+            o.step(1)
+            err = rel_l2(es.get_state(ss[k])[0], o.get_state()[0])
+            assert err <= 2e-6, (n, err)
+        for eng, _ in engines:
+            eng.step(12)
+        for k in range(len(specs)):
+            if k == 1:
+                continue
+            wa, ma = es.get_state(ss[k])
+            wb, mb = ef.get_state(sf[k])
+            assert np.array_equal(wa, wb) and np.array_equal(ma, mb), k
+            assert np.array_equal(es.energy(ss[k], 13), ef.energy(sf[k], 13)), k
+        for eng, _ in engines:
+            eng.close()
+    finally:
+        for n in MIX:
+            af.select_schedule(32, n, n, -1)
