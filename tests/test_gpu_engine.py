@@ -219,3 +219,48 @@ def test_single_launch_mix_bitwise_and_oracle(af, orc):
     finally:
         for n in MIX:
             af.select_schedule(32, n, n, -1)
+
+
+def test_descriptor_array_growth_holes_and_compaction(af):
+    """Membership changes edit a class's descriptor array in place: 20 runs of
+    one size (the array grows past its first 16 entries), then 7 removed (holes
+    beyond a quarter of the entries: the next step compacts, moving runs to
+    other entries) and 2 added (filling holes, reusing retired blocks).  Every
+    surviving run equals the same run stepped alone, bitwise, and the bytes
+    held follow the membership."""
+    n = 64
+    rng = np.random.default_rng(21)
+    specs = []
+    for i in range(22):
+        p = af.params(n, dt=2e-4, lam=float(rng.uniform(1, 5)), alpha=float(rng.uniform(-1.2, -0.4)),
+                      beta=float(rng.uniform(0.005, 0.02)), gamma0=-0.045, gamma2=2.5e-5, precision=32)
+        specs.append((p, af.ic_bandlimited(n, seed=100 + i, amp=1.0, kmax=8.0)))
+    per_run = af.device_bytes(specs[0][0])
+    eng = af.Engine(1 << 32)
+    slots = {i: eng.add(*specs[i]) for i in range(20)}
+    eng.step(3)
+    steps = {i: 3 for i in slots}
+    for i in (0, 3, 4, 9, 12, 17, 19):
+        eng.remove(slots.pop(i))
+        steps.pop(i)
+    assert eng.count == 13 and eng.bytes_in_use == 13 * per_run
+    eng.step(4)                      # compaction happens here
+    for i in steps:
+        steps[i] += 4
+    for i in (20, 21):
+        slots[i] = eng.add(*specs[i])
+        steps[i] = 0
+    assert eng.count == 15 and eng.bytes_in_use == 15 * per_run
+    eng.step(5)
+    for i in steps:
+        steps[i] += 5
+    for i, s in slots.items():
+        solo = af.Engine(1 << 30)
+        s1 = solo.add(*specs[i])
+        solo.step(steps[i])
+        wa, ma = solo.get_state(s1)
+        wm, mm = eng.get_state(s)
+        assert eng.steps_done(s) == steps[i]
+        assert np.array_equal(wa, wm) and np.array_equal(ma, mm), i
+        solo.close()
+    eng.close()
