@@ -189,11 +189,13 @@ af_status nyquist_check(const Geometry& g, const double* omega_hat) {
 }
 
 // Reference-layout state -> bricks + mean, then the stage-0 column pass.
+// `keep` (optional): a device buffer of state_staging_doubles(g) doubles to
+// use instead of a temporary allocation.
 af_status load_state(const Geometry& g, unsigned char* base, const double* omega_hat, const double M[2],
-                     hipStream_t stream) {
-  const size_t n = (size_t)g.ny * (g.nx2 + 1) * 2;
-  double* staging = nullptr;
-  hipError_t e = hipMalloc(&staging, n * sizeof(double));
+                     hipStream_t stream, double* keep) {
+  const size_t n = state_staging_doubles(g);
+  double* staging = keep;
+  hipError_t e = keep ? hipSuccess : hipMalloc(&staging, n * sizeof(double));
   if (e != hipSuccess) return fail(AF_ENOMEM, std::string("staging: ") + hipGetErrorString(e));
   af_status st = AF_OK;
   MeanState ms;
@@ -218,14 +220,15 @@ af_status load_state(const Geometry& g, unsigned char* base, const double* omega
     }
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) { st = hip_fail(e, "set_state sync"); break; }
   } while (0);
-  (void)hipFree(staging);
+  if (!keep) (void)hipFree(staging);
   return st;
 }
 
-af_status store_state(const Geometry& g, unsigned char* base, double* omega_hat, double M[2], hipStream_t stream) {
-  const size_t n = (size_t)g.ny * (g.nx2 + 1) * 2;
-  double* staging = nullptr;
-  hipError_t e = hipMalloc(&staging, n * sizeof(double));
+af_status store_state(const Geometry& g, unsigned char* base, double* omega_hat, double M[2], hipStream_t stream,
+                      double* keep) {
+  const size_t n = state_staging_doubles(g);
+  double* staging = keep;
+  hipError_t e = keep ? hipSuccess : hipMalloc(&staging, n * sizeof(double));
   if (e != hipSuccess) return fail(AF_ENOMEM, std::string("staging: ") + hipGetErrorString(e));
   af_status st = AF_OK;
   MeanState ms;
@@ -242,7 +245,7 @@ af_status store_state(const Geometry& g, unsigned char* base, double* omega_hat,
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) { st = hip_fail(e, "get_state sync"); break; }
     if (M) { M[0] = ms.M[0]; M[1] = ms.M[1]; }
   } while (0);
-  (void)hipFree(staging);
+  if (!keep) (void)hipFree(staging);
   return st;
 }
 
@@ -283,6 +286,7 @@ struct af_run {
   hipStream_t stream = nullptr;
   unsigned char* base = nullptr;
   HostChunk* host = nullptr;        // pinned
+  double* staging = nullptr;        // reference-layout state for set/get, allocated on first use
   hipGraph_t graph[kGraphLevels] = {};       // graph[l]: 2^l steps
   hipGraphExec_t gexec[kGraphLevels] = {};
   bool use_graph = true;
@@ -291,6 +295,8 @@ struct af_run {
   long long steps = 0;
   int device = 0;                   // the HIP device current at creation
 };
+
+static af_status prepare_graphs(af_run* r);
 
 extern "C" {
 
@@ -348,8 +354,22 @@ af_status af_run_create(const af_params* p, af_run** out) {
     return fail(AF_ENOMEM, std::string("pinned host buffer: ") + hipGetErrorString(e));
   }
   st = init_device_block(p, r->g, r->base, r->stream);
+  // the step graphs depend only on the run's buffers: captured, instantiated
+  // and uploaded once, here, so that nothing of that host work sits between a
+  // caller's set_state and its first steps
+  if (st == AF_OK) st = prepare_graphs(r);
   if (st != AF_OK) { af_run_destroy(r); return st; }
   *out = r;
+  return AF_OK;
+}
+
+static af_status ensure_staging(af_run* r) {
+  if (r->staging) return AF_OK;
+  hipError_t e = hipMalloc(&r->staging, state_staging_doubles(r->g) * sizeof(double));
+  if (e != hipSuccess) {
+    r->staging = nullptr;
+    return fail(AF_ENOMEM, std::string("staging: ") + hipGetErrorString(e));
+  }
   return AF_OK;
 }
 
@@ -360,8 +380,8 @@ af_status af_run_set_state(af_run* r, const double* omega_hat, const double M[2]
   r->ready = false;
   r->steps = 0;
   af_status st = nyquist_check(r->g, omega_hat);
-  if (st == AF_OK) st = load_state(r->g, r->base, omega_hat, M, r->stream);
-  if (st == AF_OK) st = prepare_graphs(r);
+  if (st == AF_OK) st = ensure_staging(r);
+  if (st == AF_OK) st = load_state(r->g, r->base, omega_hat, M, r->stream, r->staging);
   r->ready = (st == AF_OK);
   return st;
 }
@@ -369,7 +389,9 @@ af_status af_run_set_state(af_run* r, const double* omega_hat, const double M[2]
 af_status af_run_get_state(af_run* r, double* omega_hat, double M[2]) {
   if (!r || !omega_hat || !M) return fail(AF_EINVAL, "NULL argument");
   if (!r->ready) return fail(AF_EINVAL, "state not set");
-  return store_state(r->g, r->base, omega_hat, M, r->stream);
+  af_status st = ensure_staging(r);
+  if (st != AF_OK) return st;
+  return store_state(r->g, r->base, omega_hat, M, r->stream, r->staging);
 }
 
 // Captures `steps` steps into (graph, exec); false (and nothing kept) on failure.
@@ -573,6 +595,7 @@ void af_run_destroy(af_run* r) {
   }
   if (r->base) (void)hipFree(r->base);
   if (r->host) (void)hipHostFree(r->host);
+  if (r->staging) (void)hipFree(r->staging);
   if (r->stream) (void)hipStreamDestroy(r->stream);
   delete r;
 }

@@ -36,9 +36,12 @@ af_status validate(const af_params* p);
 Geometry geometry(const af_params* p, bool batched = false);
 af_status init_device_block(const af_params* p, const Geometry& g, unsigned char* base, hipStream_t stream);
 af_status nyquist_check(const Geometry& g, const double* omega_hat);
+// Doubles of the reference-layout state (the staging buffer of set/get).
+inline size_t state_staging_doubles(const Geometry& g) { return (size_t)g.ny * (g.nx2 + 1) * 2; }
 af_status load_state(const Geometry& g, unsigned char* base, const double* omega_hat, const double M[2],
-                     hipStream_t stream);
-af_status store_state(const Geometry& g, unsigned char* base, double* omega_hat, double M[2], hipStream_t stream);
+                     hipStream_t stream, double* keep = nullptr);
+af_status store_state(const Geometry& g, unsigned char* base, double* omega_hat, double M[2], hipStream_t stream,
+                      double* keep = nullptr);
 hipError_t launch_step(const Geometry& g, const void* descs, int ninst, const void* pool, hipStream_t s);
 af_status diagnostics_noalloc(const af_params& p, const Geometry& g, unsigned char* base, hipStream_t stream, int device,
                               double* energy, double* enstrophy, double* spectrum, int nshell);

@@ -480,10 +480,16 @@ def main():
     if args.batch > 0:
         print(json.dumps({"batched": batched_leg(af, N, args.batch, args.steps)}))
         return
-    # The other single-run workloads go first: the headline's W warm-up steps
-    # then follow other GPU work, as in a process that has been stepping, instead
-    # of being the first GPU work of the process (its first ~2 ms run ~8 %
-    # slower while the device comes out of idle).
+    # Device warm-up, untimed: after idle the device needs ~200 steps of work
+    # (about 20 ms at 1024^2) before it steps at full speed (measured: a 20-step
+    # call right after other GPU work takes 1.88 ms, after 100 ms of idle 1.95
+    # ms).  Every leg's own W warm-up steps then follow other GPU work, as in a
+    # process that has been stepping, and the legs compare like for like.
+    warm = af.Run(af.params(N, dt=1e-4 if N <= 1024 else 5e-5, precision=32, **PHYS))
+    warm.set_state(af.ic_bandlimited(N, seed=2, amp=1e-3, kmax=10.0))
+    warm.step(200)
+    warm.close()
+    # The other single-run workloads go next, the headline after them.
     extra = {}
     if not args.no_extra:
         for n, prec, steps in ((1024, 64, max(200, args.steps // 4)), (2048, 32, max(100, args.steps // 8)),

@@ -76,8 +76,9 @@ af_status af_run_create(const af_params* p, af_run** out);
  * The kx = Nx/2 column is not represented on this backend (every supported
  * mask removes it from the nonlinear term): a state whose kx = Nx/2 column
  * is nonzero is refused with AF_EUNSUPPORTED (dropping it would change the
- * run); zero that column to load such a state.  Also captures and uploads
- * the run's step graphs, so the first af_run_step replays resident graphs. */
+ * run); zero that column to load such a state.  (The run's step graphs are
+ * captured and uploaded by af_run_create, so no capture happens between
+ * set_state and the first steps.) */
 af_status af_run_set_state(af_run* r, const double* omega_hat, const double M[2]);
 /* Replaces the device->host copy of save_func (.cu:1259-1260). */
 af_status af_run_get_state(af_run* r, double* omega_hat, double M[2]);
@@ -113,7 +114,9 @@ void af_run_destroy(af_run* r);
 af_status af_run_profile(af_run* r, int nsteps, double* row_ms, double* col_ms, double* total_ms);
 /* Use (1) or bypass (0) a captured hipGraph for one step (default 1). */
 af_status af_run_use_graph(af_run* r, int on);
-/* Bytes of device memory held by a run with these parameters. */
+/* Bytes of device memory held by a run with these parameters (an af_run
+ * also keeps, from its first set_state or get_state, a staging buffer of
+ * 16*Ny*(Nx/2+1) bytes for the reference-layout state). */
 size_t af_run_device_bytes(const af_params* p);
 
 /* ---- schedules (no reference counterpart: cuFFT plans, .cu:504-508) ----
