@@ -273,7 +273,10 @@ struct RowPass {
   static constexpr int TR = NX2 / R;
   static constexpr int NT = RB * TR;
   using View = LdsView<T, NX2, RB, R, G, PP & 1>;
-  struct Smem {
+  // 16-byte aligned: FP64 exchanges then compile to ds_read_b128 /
+  // ds_write_b128 (with the struct's natural 8-byte alignment the compiler
+  // splits every 16-byte LDS access into two 8-byte ones)
+  struct alignas(16) Smem {
     cx<T> lds_mem[View::kElems];
     cx<T> tw_h[TwLayout<NX2, R>::kAlloc];   // the half-length transforms
     cx<T> tw_n[NX2];                        // exp(-2 pi i k / Nx), k < Nx/2, the even/odd split
@@ -502,7 +505,7 @@ struct ColPass {
   static constexpr int NT = W * TC;
   static constexpr int GMAX = GF > GI ? GF : GI;
   using View = LdsView<T, NY, W, R, GMAX, PP & 1>;
-  struct Smem {
+  struct alignas(16) Smem {   // 16-byte aligned: see RowPass::Smem
     cx<T> lds_mem[View::kElems];
     cx<T> tw_y[TwLayout<NY, R>::kAlloc];
     double mean_in[4];                       // tile 0: M, M_old
