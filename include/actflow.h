@@ -214,10 +214,12 @@ typedef struct {
 typedef void (*af_snapshot_fn)(void* user, int request, long long step, const double* omega_hat,
                                const double M[2], int nx, int ny);
 /* Runs the requests with continuous batching: admission in arrival order by
- * the HBM budget, all resident runs advanced together in ticks of at most
- * max_tick_steps (ticks end at snapshot and completion steps), snapshots
- * copied back asynchronously, finished runs retired so waiting requests can
- * enter.  final_states (optional) holds per request NULL or a buffer of
+ * the HBM budget; the resident runs of one size class advance together in
+ * chunks of at most max_tick_steps (a chunk ends at the next snapshot or
+ * completion step of any of the class's runs), each class on its own as soon
+ * as its previous chunk completes; snapshots copied back asynchronously,
+ * finished runs retired so waiting requests can enter.  stats->ticks counts
+ * the chunks of all classes.  final_states (optional) holds per request NULL or a buffer of
  * 2*Ny*(Nx/2+1) doubles for the final state.  A request's result is bitwise
  * the result of running it alone.  A request whose state goes NaN fails alone
  * (result status AF_ENAN, finish_s -1, the check_func stop of .cu:850-861);
