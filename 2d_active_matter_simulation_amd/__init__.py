@@ -251,8 +251,9 @@ class Run:
         _check(lib().af_run_use_graph(self._h, 1 if on else 0))
 
     def profile(self, nsteps: int):
-        """Mean row-pass / column-pass kernel time (ms) over nsteps, timed with
-        HIP events on the run's stream, and the total wall of the region."""
+        """Mean row-pass / column-pass kernel time (ms) over nsteps steps timed
+        with HIP events on the run's stream, each after a 4-step graph (so the
+        run advances 5 * nsteps steps), and the total wall of the region."""
         r, c, t = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         _check(lib().af_run_profile(self._h, nsteps, ctypes.byref(r), ctypes.byref(c), ctypes.byref(t)))
         return r.value, c.value, t.value

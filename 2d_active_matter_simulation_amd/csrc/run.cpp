@@ -508,7 +508,13 @@ af_status af_run_use_graph(af_run* r, int on) {
 af_status af_run_profile(af_run* r, int nsteps, double* row_ms, double* col_ms, double* total_ms) {
   if (!r || nsteps <= 0) return fail(AF_EINVAL, "bad arguments");
   if (!r->ready) return fail(AF_EINVAL, "state not set");
-  // events at each kernel's own start and end, plus two around the whole run
+  // events at each kernel's own start and end, plus two around the whole run.
+  // Every timed step follows a graph of kLead steps: enqueueing a launch with
+  // its event pair takes the host longer than the kernel takes the device, so
+  // timed launches alone would leave the device idle between kernels; behind
+  // the graph they run back to back, as the graph-launched steps of
+  // af_run_step do.
+  constexpr int kLeadLevel = 2, kLead = 1 << kLeadLevel;
   const int nl = 8 * nsteps;
   std::vector<hipEvent_t> ev(2 * nl + 2, nullptr);
   for (auto& x : ev)
@@ -521,7 +527,9 @@ af_status af_run_profile(af_run* r, int nsteps, double* row_ms, double* col_ms, 
   const void* pool = r->base + r->g.off_pool;
   hipError_t e = hipEventRecord(ev[2 * nl], r->stream);
   int k = 0;
-  for (int s = 0; s < nsteps && e == hipSuccess; ++s)
+  for (int s = 0; s < nsteps && e == hipSuccess; ++s) {
+    g_launch_events = {};   // the lead steps record no events (plain launches when graphs are off)
+    e = run_steps(r, kLeadLevel);
     for (int stage = 1; stage <= 4 && e == hipSuccess; ++stage) {
       g_launch_events = {ev[2 * k], ev[2 * k + 1]};
       ++k;
@@ -530,6 +538,7 @@ af_status af_run_profile(af_run* r, int nsteps, double* row_ms, double* col_ms, 
       ++k;
       if (e == hipSuccess) e = r->g.cc->launch(desc, 1, r->g.col_tiles, stage, pool, r->stream);
     }
+  }
   g_launch_events = {};
   if (e == hipSuccess) e = hipEventRecord(ev[2 * nl + 1], r->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
@@ -546,9 +555,11 @@ af_status af_run_profile(af_run* r, int nsteps, double* row_ms, double* col_ms, 
       if (row_ms) *row_ms = rs / (4.0 * nsteps);
       if (col_ms) *col_ms = cs / (4.0 * nsteps);
       if (total_ms) *total_ms = tot;
-      r->steps += nsteps;
+      r->steps += (long long)nsteps * (kLead + 1);
     }
   }
+  // some steps may have run: the host step count no longer matches the device's
+  if (e != hipSuccess) r->ready = false;
   for (auto& x : ev) (void)hipEventDestroy(x);
   if (e != hipSuccess) return hip_fail(e, "profile");
   return AF_OK;

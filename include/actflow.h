@@ -106,10 +106,12 @@ af_status af_run_diagnostics(af_run* r, double* energy, double* enstrophy, doubl
 af_status af_run_diagnostics_noalloc(af_run* r, double* energy, double* enstrophy, double* spectrum, int nshell);
 void af_run_destroy(af_run* r);
 
-/* Per-kernel timing with HIP events on the run's stream: runs nsteps steps,
- * each launch recording an event pair at its kernel's own start and end
- * (hipExtLaunchKernel), and returns the mean duration in milliseconds of the
- * row pass and of the column pass (stages 1-4), and the wall time of the whole
+/* Per-kernel timing with HIP events on the run's stream: runs 5*nsteps steps,
+ * nsteps of them with each launch recording an event pair at its kernel's own
+ * start and end (hipExtLaunchKernel), each of those after a 4-step graph so
+ * that the device runs the timed kernels back to back as in af_run_step, and
+ * returns the mean duration in milliseconds of the row pass and of the column
+ * pass (stages 1-4) over the timed steps, and the wall time of the whole
  * region.  No reference counterpart (measurement hook for bench.py). */
 af_status af_run_profile(af_run* r, int nsteps, double* row_ms, double* col_ms, double* total_ms);
 /* Use (1) or bypass (0) a captured hipGraph for one step (default 1). */
