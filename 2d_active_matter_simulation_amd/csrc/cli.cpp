@@ -4,11 +4,15 @@
 // codes; the time step runs on the GPU through libactflow.so.
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <fstream>
 #include <iostream>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/actflow.h"
@@ -62,6 +66,128 @@ std::string fmt(double v) {
   std::snprintf(b, sizeof b, "%g", v);
   return b;
 }
+
+// The files of a run are written by one background thread, in the order the
+// jobs are submitted, while the GPU steps the next chunk: energy lines
+// (appended to energy.txt, .cu:1268-1278) and snapshots (.cu:1245-1266).
+// After a job fails no later job is written, so the files on disk are those
+// of a run that stopped at that job, as the reference does; the main loop
+// sees the failure at its next chunk and exits with the job's message.
+class Writer {
+ public:
+  Writer() : th_([this] { loop(); }) {}
+  ~Writer() { finish(); }
+
+  // the energies of steps m .. m + n - 1 (step m labelled m*dt)
+  void energy(const std::string& path, long long m, float dt, const double* e, int n) {
+    Job j;
+    j.snapshot = false;
+    j.path = path;
+    j.m = m;
+    j.dt = dt;
+    j.data.assign(e, e + n);
+    push(std::move(j));
+  }
+  // waits while two snapshots are queued (bounds the memory held)
+  void snapshot(const std::string& path, int nx, int ny, const std::vector<double>& w, const double M[2],
+                int precision) {
+    std::unique_lock<std::mutex> lk(mu_);
+    space_.wait(lk, [&] { return queued_snaps_ < 2 || failed_; });
+    lk.unlock();
+    Job j;
+    j.snapshot = true;
+    j.path = path;
+    j.nx = nx;
+    j.ny = ny;
+    j.precision = precision;
+    j.M[0] = M[0];
+    j.M[1] = M[1];
+    j.data = w;
+    push(std::move(j));
+  }
+  // the message of the first failed job, or "" if none failed so far
+  std::string failure() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return msg_;
+  }
+  // writes everything queued, stops the thread; returns failure()
+  std::string finish() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      done_ = true;
+    }
+    work_.notify_one();
+    if (th_.joinable()) th_.join();
+    return failure();
+  }
+
+ private:
+  struct Job {
+    bool snapshot = false;
+    std::string path;
+    long long m = 0;
+    float dt = 0;
+    int nx = 0, ny = 0, precision = 6;
+    double M[2] = {0, 0};
+    std::vector<double> data;
+  };
+
+  void push(Job&& j) {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      if (failed_) return;
+      if (j.snapshot) ++queued_snaps_;
+      q_.push_back(std::move(j));
+    }
+    work_.notify_one();
+  }
+
+  void loop() {
+    for (;;) {
+      Job j;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        work_.wait(lk, [&] { return !q_.empty() || done_; });
+        if (q_.empty()) return;
+        j = std::move(q_.front());
+        q_.pop_front();
+      }
+      const std::string err = j.snapshot ? write_snapshot(j) : write_energy(j);
+      std::lock_guard<std::mutex> lk(mu_);
+      if (j.snapshot) --queued_snaps_;
+      if (!err.empty() && !failed_) {
+        failed_ = true;
+        msg_ = err;
+        for (auto& x : q_)
+          if (x.snapshot) --queued_snaps_;
+        q_.clear();
+      }
+      space_.notify_all();
+    }
+  }
+
+  static std::string write_energy(const Job& j) {
+    FILE* ef = std::fopen(j.path.c_str(), "a");
+    if (!ef) return "Error: failed to save the mean energy";
+    for (size_t i = 0; i < j.data.size(); ++i)
+      std::fprintf(ef, "%s %s\n", fmt((float)(j.m + (long long)i) * j.dt).c_str(), fmt(j.data[i]).c_str());
+    return std::fclose(ef) == 0 ? "" : "Error: failed to save the mean energy";
+  }
+
+  static std::string write_snapshot(const Job& j) {
+    return af_snapshot_write(j.path.c_str(), j.nx, j.ny, j.data.data(), j.M, j.precision) == AF_OK
+               ? ""
+               : "Error: unable to create snapshot file";
+  }
+
+  std::mutex mu_;
+  std::condition_variable work_, space_;
+  std::deque<Job> q_;
+  int queued_snaps_ = 0;
+  bool done_ = false, failed_ = false;
+  std::string msg_;
+  std::thread th_;
+};
 
 }  // namespace
 
@@ -189,15 +315,12 @@ int main(int argc, char* argv[]) {
     std::cerr << "Error: unknown specification for the initial condition" << std::endl;
     return 1;
   }
+  Writer writer;   // files are written in order on its thread while the GPU steps
   if (m == 1) {
     std::cout << "processing the initial condition" << std::endl;
     w[0] = 0;
     w[1] = 0;
-    if (af_snapshot_write(snap_name(directory, 0).c_str(), spec.Nx, spec.Ny, w.data(), Mv,
-                          spec.snapshot_precision) != AF_OK) {
-      std::cerr << "Error: unable to create snapshot file" << std::endl;
-      return 1;
-    }
+    writer.snapshot(snap_name(directory, 0), spec.Nx, spec.Ny, w, Mv, spec.snapshot_precision);
   }
 
   af_run* run = nullptr;
@@ -208,11 +331,19 @@ int main(int argc, char* argv[]) {
     af_run_destroy(run);
     return 1;
   }
+  // a failed write ends the run with the reference's message for that file
+  auto write_failed = [&](std::string msg) {
+    if (msg.empty()) return false;
+    std::cerr << msg << std::endl;
+    af_run_destroy(run);
+    return true;
+  };
   std::cout << "initializing the time stepping" << std::endl;
   const auto t0 = std::chrono::high_resolution_clock::now();
   const std::string energy_path = directory + "/energy.txt";
   std::vector<double> energy;
   while (m <= M) {
+    if (write_failed(writer.failure())) return 1;
     // chunk up to the next snapshot step (inclusive) or the end
     long long next = ((m + snap - 1) / snap) * snap;
     if (next < m) next = m;
@@ -222,16 +353,9 @@ int main(int argc, char* argv[]) {
     long long nan_step = -1;
     st = af_run_step(run, k, energy.data(), &nan_step);
     const int good = (st == AF_ENAN) ? (int)(nan_step - (af_run_steps_done(run) - k)) : k;
-    FILE* ef = std::fopen(energy_path.c_str(), "a");
-    if (!ef) {
-      std::cerr << "Error: failed to save the mean energy" << std::endl;
-      af_run_destroy(run);
-      return 1;
-    }
-    for (int i = 0; i < good && i < k; ++i)
-      std::fprintf(ef, "%s %s\n", fmt((float)(m + i) * dt).c_str(), fmt(energy[i]).c_str());
-    std::fclose(ef);
+    writer.energy(energy_path, m, dt, energy.data(), good < k ? good : k);
     if (st == AF_ENAN) {
+      if (write_failed(writer.finish())) return 1;
       std::cerr << "\nError: array contains nan values after " << (m + good - 1) << " timesteps" << std::endl;
       af_run_destroy(run);
       return 1;
@@ -244,18 +368,19 @@ int main(int argc, char* argv[]) {
     m += k;
     const long long last = m - 1;
     if (last % snap == 0) {
-      if (af_run_get_state(run, w.data(), Mv) != AF_OK ||
-          af_snapshot_write(snap_name(directory, last / snap).c_str(), spec.Nx, spec.Ny, w.data(), Mv,
-                            spec.snapshot_precision) != AF_OK) {
+      if (af_run_get_state(run, w.data(), Mv) != AF_OK) {
+        writer.finish();
         std::cerr << "Error: unable to create snapshot file" << std::endl;
         af_run_destroy(run);
         return 1;
       }
+      writer.snapshot(snap_name(directory, last / snap), spec.Nx, spec.Ny, w, Mv, spec.snapshot_precision);
       const std::chrono::duration<double> el = std::chrono::high_resolution_clock::now() - t0;
       std::cout << "\r" << last << " / " << M << " timesteps done in " << el.count() << "s, w[0]=("
                 << (float)w[0] << "," << (float)w[1] << ")    " << std::flush;
     }
   }
+  if (write_failed(writer.finish())) return 1;
   af_run_destroy(run);
   const std::chrono::duration<double> el = std::chrono::high_resolution_clock::now() - t0;
   std::cout << "\nprogram finished successfully after " << el.count() << "s.\n" << std::endl;

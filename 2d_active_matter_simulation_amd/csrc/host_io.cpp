@@ -3,6 +3,7 @@
 // snapshot text files (save_func .cu:1245-1266, reader .cu:615-634),
 // energy.txt lines (.cu:1268-1278) and the initial conditions (.cu:535-566,
 // 660-679; the band-limited seeded IC has no reference counterpart).
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <complex>
@@ -13,6 +14,8 @@
 #include <random>
 #include <sstream>
 #include <string>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "status.hpp"
@@ -180,20 +183,42 @@ af_status af_snapshot_write(const char* path, int nx, int ny, const double* w, c
   FILE* f = std::fopen(path, "wb");
   if (!f) return fail(AF_EIO, std::string("cannot open ") + path);
   const size_t nc = (size_t)ny * (nx / 2 + 1);
-  std::string buf;
-  buf.reserve(1 << 20);
-  char b[64];
-  for (size_t c = 0; c < nc; ++c) {
-    int n = std::snprintf(b, sizeof b, "%.*g,%.*g,", precision, w[2 * c], precision, w[2 * c + 1]);
-    buf.append(b, (size_t)n);
-    if (buf.size() > (1u << 20)) {
-      if (std::fwrite(buf.data(), 1, buf.size(), f) != buf.size()) { std::fclose(f); return fail(AF_EIO, "write"); }
-      buf.clear();
+  // Formatting (a %g per value, ~4 M values at 2048^2) costs far more than
+  // the write: rounds of up to 16 blocks of kBlock modes are formatted by one
+  // thread each and written in block order, so the bytes are those of one
+  // sequential pass.
+  constexpr size_t kBlock = 1 << 16;
+  const size_t nblocks = (nc + kBlock - 1) / kBlock;
+  const unsigned hw = std::thread::hardware_concurrency();
+  const size_t nthreads = std::min<size_t>(std::min<size_t>(hw ? hw : 1, 16), nblocks);
+  std::vector<std::string> text(nthreads);
+  auto format = [&](size_t blk, std::string& out) {
+    out.clear();
+    char b[64];
+    const size_t c1 = std::min(nc, (blk + 1) * kBlock);
+    for (size_t c = blk * kBlock; c < c1; ++c) {
+      const int n = std::snprintf(b, sizeof b, "%.*g,%.*g,", precision, w[2 * c], precision, w[2 * c + 1]);
+      out.append(b, (size_t)n);
     }
+  };
+  bool ok = true;
+  for (size_t b0 = 0; b0 < nblocks && ok; b0 += nthreads) {
+    const size_t nb = std::min(nthreads, nblocks - b0);
+    std::vector<std::thread> pool;
+    size_t started = 1;   // blocks b0 + 1 .. b0 + started - 1 run on threads
+    try {
+      for (; started < nb; ++started) pool.emplace_back(format, b0 + started, std::ref(text[started]));
+    } catch (const std::system_error&) {
+      // no more threads: the rest of the round is formatted here
+    }
+    format(b0, text[0]);
+    for (size_t i = started; i < nb; ++i) format(b0 + i, text[i]);
+    for (auto& t : pool) t.join();
+    for (size_t i = 0; i < nb && ok; ++i) ok = std::fwrite(text[i].data(), 1, text[i].size(), f) == text[i].size();
   }
-  int n = std::snprintf(b, sizeof b, "%.*g,%.*g", precision, M[0], precision, M[1]);
-  buf.append(b, (size_t)n);
-  const bool ok = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+  char b[64];
+  const int n = std::snprintf(b, sizeof b, "%.*g,%.*g", precision, M[0], precision, M[1]);
+  ok = ok && std::fwrite(b, 1, (size_t)n, f) == (size_t)n;
   return (std::fclose(f) == 0 && ok) ? AF_OK : fail(AF_EIO, "write");
 }
 

@@ -57,6 +57,17 @@ int main(int argc, char** argv) {
     CHECK(af_snapshot_read(path.c_str(), 2 * nx, ny, big.data(), Mr) != AF_OK);   // too short for the grid
     CHECK(af_snapshot_read((scratch + "/missing.bin").c_str(), nx, ny, r.data(), Mr) != AF_OK);
   }
+  {   // a state formatted in several blocks on several threads (512 x 512: 131584 modes)
+    const int nx = 512, ny = 512;
+    const size_t n = 2 * (size_t)ny * (nx / 2 + 1);
+    std::vector<double> w(n), r(n);
+    double M[2] = {0.125, -3.5e-7}, Mr[2];
+    for (size_t i = 0; i < n; ++i) w[i] = std::sin(0.731 * (double)i) * std::pow(10.0, (double)(i % 25) - 12.0);
+    const std::string path = scratch + "/snap_big.bin";
+    CHECK(af_snapshot_write(path.c_str(), nx, ny, w.data(), M, 17) == AF_OK);
+    CHECK(af_snapshot_read(path.c_str(), nx, ny, r.data(), Mr) == AF_OK);
+    CHECK(std::memcmp(w.data(), r.data(), n * sizeof(double)) == 0 && Mr[0] == M[0] && Mr[1] == M[1]);
+  }
   CHECK(af_energy_append((scratch + "/energy.txt").c_str(), 0.005, 1.25) == AF_OK);
 
   // oracle: transforms and a few steps of both precisions, square and not

@@ -91,6 +91,24 @@ def test_snapshot_default_precision_is_ostream(af, tmp_path):
     assert np.allclose(w2, [[1.23457e6 + 0.000123457j, -0.5, 3 + 1e-20j]])
 
 
+@pytest.mark.parametrize("precision", [6, 17])
+def test_snapshot_bytes_of_a_multi_block_state(af, tmp_path, precision):
+    """A snapshot large enough to be formatted in several blocks on several
+    threads (512^2: 131584 modes, blocks of 65536) is byte for byte the
+    sequential %g text (.cu:1261-1263): every value, comma and the order."""
+    rng = np.random.default_rng(precision)
+    ny, nx = 512, 512
+    w = (rng.standard_normal((ny, nx // 2 + 1)) * 10.0 ** rng.integers(-12, 12, (ny, nx // 2 + 1))
+         + 1j * rng.standard_normal((ny, nx // 2 + 1)))
+    w[3, 4] = -0.0
+    M = (0.125, -3.5e-7)
+    path = tmp_path / "big.bin"
+    af.snapshot_write(path, w, M, precision=precision)
+    vals = np.column_stack([w.real.ravel(), w.imag.ravel()]).ravel().tolist() + list(M)
+    expect = ",".join("%.*g" % (precision, v) for v in vals)
+    assert path.read_text() == expect
+
+
 def test_snapshot_image_real_field(af, tmp_path):
     """Real-space image (image.py:67): stored values are unnormalised DFT
     coefficients, so a single stored coefficient Nx*Ny/2 at kx = 1 is cos(x)

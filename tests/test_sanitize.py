@@ -18,7 +18,6 @@ def test_host_code_under_asan_ubsan(tmp_path):
     subprocess.run(["make", "-s", "-C", HERE, f"OUT={tmp_path}"], check=True)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1",
                OMP_NUM_THREADS="1")
-    env.pop("LD_PRELOAD", None)
     r = subprocess.run([str(tmp_path / "check_sanitized"), GOLDEN, str(tmp_path)], capture_output=True, text=True,
                        env=env, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
