@@ -315,13 +315,18 @@ int main(int argc, char* argv[]) {
     std::cerr << "Error: unknown specification for the initial condition" << std::endl;
     return 1;
   }
-  Writer writer;   // files are written in order on its thread while the GPU steps
   if (m == 1) {
+    // written before the run is created, as the reference does (.cu:682-693)
     std::cout << "processing the initial condition" << std::endl;
     w[0] = 0;
     w[1] = 0;
-    writer.snapshot(snap_name(directory, 0), spec.Nx, spec.Ny, w, Mv, spec.snapshot_precision);
+    if (af_snapshot_write(snap_name(directory, 0).c_str(), spec.Nx, spec.Ny, w.data(), Mv,
+                          spec.snapshot_precision) != AF_OK) {
+      std::cerr << "Error: unable to create snapshot file" << std::endl;
+      return 1;
+    }
   }
+  Writer writer;   // later files are written in order on its thread while the GPU steps
 
   af_run* run = nullptr;
   af_status st = af_run_create(&p, &run);

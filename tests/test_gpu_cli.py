@@ -91,6 +91,27 @@ def test_cli_run_and_continue(af, orc, tmp_path):
     assert np.allclose(M7, mo7, rtol=1e-11, atol=1e-11)
 
 
+def test_cli_snapshot_write_failure_mid_run(tmp_path):
+    """Snapshot 2 cannot be created (a directory of that name is in the way):
+    the run stops with the reference's message and exit code 1, and the files
+    are those of a run that stopped there: energy lines up to step 2*snap,
+    snapshots 0 and 1, nothing after (the writer thread drops later jobs)."""
+    d = tmp_path / "sim"
+    d.mkdir()
+    (d / "specifications.txt").write_text(SPEC.format(T="0.05", ic="bandlimited 3 0.5 8"))
+    (d / "ActTurb_snap2.bin").mkdir()
+    r = subprocess.run([CLI, str(d)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1
+    assert "Error: unable to create snapshot file" in r.stderr
+    assert (d / "ActTurb_snap0.bin").is_file() and (d / "ActTurb_snap1.bin").is_file()
+    for j in (3, 4, 5):
+        assert not (d / f"ActTurb_snap{j}.bin").exists()
+    # jobs after the failed one are never written, whenever the main loop
+    # notices the failure: exactly the 20 lines before snapshot 2
+    lines = (d / "energy.txt").read_text().splitlines()
+    assert len(lines) == 20 and np.isclose(float(lines[-1].split()[0]), 0.02)
+
+
 def test_cli_nan_exit(tmp_path):
     d = tmp_path / "sim"
     d.mkdir()

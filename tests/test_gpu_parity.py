@@ -137,6 +137,25 @@ def test_graph_and_direct_launch_identical(af):
     assert np.array_equal(wa, wb) and np.array_equal(ma, mb) and np.array_equal(ea, eb)
 
 
+def test_profiled_steps_are_ordinary_steps(af):
+    """The per-kernel timing hook (each event-timed step behind a 4-step
+    graph) advances the run by 5 steps per timed step, with the results of
+    ordinary steps, bitwise; the energy ring and the step count follow."""
+    p = af.params(128, dt=2e-4, precision=32, **JW)
+    w0 = af.ic_bandlimited(128, seed=5, amp=1.0)
+    a, b = af.Run(p), af.Run(p)
+    for r in (a, b):
+        r.set_state(w0, (3.0, 1.0))
+    row_ms, col_ms, tot_ms = a.profile(3)
+    assert a.steps_done == 15 and row_ms > 0 and col_ms > 0 and tot_ms > 0
+    b.step(15)
+    ea, eb = a.step(7, energy=True), b.step(7, energy=True)
+    (wa, ma), (wb, mb) = a.get_state(), b.get_state()
+    a.close()
+    b.close()
+    assert np.array_equal(wa, wb) and np.array_equal(ma, mb) and np.array_equal(ea, eb)
+
+
 def test_state_roundtrip_and_nyquist(af):
     """set/get round trip; a state with a nonzero kx = Nx/2 column (not
     represented on this backend) is refused, not silently changed."""

@@ -231,3 +231,9 @@ def test_cli_failure_injection(tmp_path):
     (d / "specifications.txt").write_text("T:\n1\ndt:\n0.1\nNx:\n64\nNy:\n64\ninitial_condition:\ncontinue\n")
     r = run_cli([str(d)], tmp_path)
     assert r.returncode == 1 and "no previous snapshot was found" in r.stderr
+    # snapshot 0 cannot be written (save_directory missing): reported before any run is created
+    (d / "specifications.txt").write_text("T:\n1\ndt:\n0.1\nNx:\n64\nNy:\n64\ninitial_condition:\nblank\n"
+                                          f"save_directory:\n{tmp_path / 'missing'}\n")
+    r = run_cli([str(d)], tmp_path)
+    assert r.returncode == 1 and "Error: unable to create snapshot file" in r.stderr
+    assert "initializing the time stepping" not in r.stdout
