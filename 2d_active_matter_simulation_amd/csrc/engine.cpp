@@ -110,8 +110,10 @@ static af_status class_insert(af_engine* e, Class& c, int id) {
       if ((he = hipMalloc(&bigger, (size_t)2 * c.cap * c.desc_bytes)) != hipSuccess)
         return fail(AF_ENOMEM, "descriptor array");
       if ((he = hipMemcpyAsync(bigger, c.descs, (size_t)c.cap * c.desc_bytes, hipMemcpyDeviceToDevice, c.stream)) !=
-          hipSuccess)
+          hipSuccess) {
+        (void)hipFree(bigger);
         return hip_fail(he, "descriptor array copy");
+      }
       e->graveyard.push_back(c.descs);
       c.descs = bigger;
       c.cap *= 2;
