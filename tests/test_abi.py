@@ -107,3 +107,21 @@ def test_autotune_merge_keeps_untuned_lines():
     new = at.merge_table(["# header", "32 1024 1024 4 2 4 3 4 2 1 2 15   # 0.09"], old)
     rows = [ln for ln in new.splitlines() if not ln.startswith("#")]
     assert rows == ["32 1024 1024 4 2 4 3 4 2 1 2 15   # 0.09", "64 1024 1024 4 2 4 3 4 2 3 2 0   # 0.2"]
+
+
+def build_c_example(out_dir):
+    """tests/c_abi/step_loop.c: a plain C99 caller of the ABI (strict flags,
+    warnings as errors), linked against the library."""
+    exe = os.path.join(str(out_dir), "step_loop")
+    lib_dir = os.path.join(ROOT, "2d_active_matter_simulation_amd")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror",
+                    "-I", os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "c_abi", "step_loop.c"),
+                    "-L", lib_dir, "-lactflow", f"-Wl,-rpath,{lib_dir}", "-o", exe], check=True)
+    return exe
+
+
+def test_header_is_strict_c99_and_links(tmp_path):
+    """The header is the boundary for C callers: a C99 program that drives
+    the reference's loop through it compiles with -pedantic -Werror and
+    links (it runs in test_gpu_c_abi.py)."""
+    assert os.path.exists(build_c_example(tmp_path))
