@@ -38,9 +38,21 @@ def lds_col(ny, W, R, GF, GI, esz, pp=0):
 # Option bits of a schedule (its last field, the kernels' PP argument):
 # 1 double-buffered LDS exchanges, 2 streaming column-pass stores, 4 streaming
 # row-pass stores, 8 streaming column-pass loads of P, Q, 16 integrating-factor
-# table at radix 16 (FP32).  The row kernel reads bits 1 and 4, the column
-# kernel 1, 2, 8 and 16.
-ROW_OPTS, COL_OPTS = 5, 27
+# table at radix 16 (FP32), 32 paired rows (two rows as one full-length complex
+# transform in the row pass).  The row kernel reads bits 1, 4 and 32, the
+# column kernel 1, 2, 8 and 16.
+ROW_OPTS, COL_OPTS = 37, 27
+# Schedules that also get a paired-rows variant (options | 32), per precision
+# and size: the default and the tuned single-run and batched choices.
+PAIRED_BASES = {
+    32: {256: [(4, 2, 4, 3, 4, 2, 3, 2, 0), (4, 2, 8, 1, 8, 1, 1, 2, 0)],
+         512: [(4, 2, 4, 3, 4, 2, 3, 1, 1), (4, 4, 8, 1, 16, 1, 1, 4, 0)],
+         1024: [(4, 2, 4, 3, 4, 2, 1, 2, 15), (4, 2, 8, 1, 8, 1, 1, 2, 0), (4, 2, 8, 3, 8, 2, 3, 2, 0)],
+         2048: [(4, 2, 16, 1, 16, 1, 1, 2, 16), (4, 4, 8, 1, 16, 1, 1, 4, 14), (4, 4, 8, 1, 16, 1, 1, 4, 0)],
+         4096: [(4, 4, 16, 1, 16, 1, 1, 2, 12)]},
+    64: {256: [(4, 2, 4, 3, 4, 2, 3, 2, 0)], 512: [(4, 2, 4, 3, 4, 2, 3, 2, 0)],
+         1024: [(4, 2, 4, 3, 4, 2, 3, 2, 0)], 2048: [(4, 2, 8, 1, 8, 1, 1, 1, 0)]},
+}
 # FP32 schedules that also get the cache-policy variants (options | 2 ... 14,
 # and | 16 ... 30 for radix-16 column passes): the fastest single-run and
 # batched schedules of each size.
@@ -114,6 +126,8 @@ def candidates(prec, n):
         for rg, gf, gi in ((1, 2, 3), (3, 2, 1), (3, 1, 1), (1, 2, 1), (1, 1, 1)):
             for o in (1, 3, 7, 11, 15):
                 alts.append((4, 2, 4, rg, 4, gf, gi, 2 if n == 1024 else 1, o))
+    for base in PAIRED_BASES.get(prec, {}).get(n, []):
+        alts.append(base[:8] + (base[8] | 32,))
     for mix in MULTI.get(prec, []):   # the schedules of the mixed launches
         alts += [s for m, s in mix if m == n]
     if prec == 32:
@@ -134,6 +148,8 @@ def feasible(prec, nx, ny, s):
         return False
     tr, tc = nx2 // rr, ny // cr
     if tr % bx or nx2 % bx or ny % by or bx % cw or tc % by:
+        return False
+    if pp & 32 and (by % 2 or rr < 2):   # paired rows: pairs of brick rows, mirrored half of each lane
         return False
     if by * tr > 512 or cw * tc > 512:
         return False
@@ -268,6 +284,7 @@ def gen_multi(outdir):
                 by, bx, rr, rg, cr, gf, gi, cw, pp = s
                 assert feasible(prec, n, n, s) and s in candidates(prec, n), (n, s)
                 assert by * (n // 2 // rr) == 256 and cw * (n // cr) == 256, (n, s)
+                assert not pp & 32, "mixed launches call the unpaired row body"
                 cls.append("{" + ", ".join(map(str, (n // 2, n, by, bx, rr, rg, cr, gf, gi, cw, pp, 0))) + "}")
                 rows.append(f"RowPass<{T}, {n // 2}, {by}, {rr}, {bx}, {rg}, {pp & ROW_OPTS}>")
                 cols.append(f"ColCfgT<{T}, {n}, {cw}, {cr}, {by}, {bx}, {gf}, {gi}, {pp & COL_OPTS}>")

@@ -32,6 +32,8 @@
 //   idx(y, x) = ((y/BY)*(Nx/2/BX) + x/BX) * BY*BX + (y%BY)*BX + x%BX,
 // so a row block reads whole bricks and a column block reads BX-wide bricks.
 #pragma once
+#include <type_traits>
+
 #include "fft_lds.hpp"
 #include "layout.hpp"
 
@@ -262,7 +264,9 @@ __device__ __forceinline__ double lds_wave_sum(const double* red, int k) {
 // R2C transforms theirs); G = 1: one field at a time (less LDS).
 // PP, option bits: 1 = two sets of LDS areas, one barrier per transform
 // exchange (more LDS); 4 = streaming stores of P, Q (otherwise device-scope
-// write-through stores).  U, V, W are read with streaming loads.
+// write-through stores); 32 = paired rows (RowPassPaired below: rows y and
+// y + RB/2 of the block as the real and imaginary parts of one full-length
+// complex transform).  U, V, W are read with streaming loads.
 // ---------------------------------------------------------------------------
 // RowPass<...>::run is the body of one block (tile `block` of the launch,
 // block size NT, LDS `sm`); row_pass is the kernel of one size class, and a
@@ -287,9 +291,12 @@ struct RowPass {
 };
 
 template <class T, int NX2, int RB, int R, int BX, int G, int PP>
+struct RowPassPaired;
+
+template <class T, int NX2, int RB, int R, int BX, int G, int PP>
 __global__ void __launch_bounds__(RB * (NX2 / R))
 row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>* __restrict__ pool) {
-  using P = RowPass<T, NX2, RB, R, BX, G, PP>;
+  using P = std::conditional_t<(PP & 32) != 0, RowPassPaired<T, NX2, RB, R, BX, G, PP>, RowPass<T, NX2, RB, R, BX, G, PP>>;
   __shared__ typename P::Smem sm;
   P::run(descs, tiles_per_inst, pool, blockIdx.x, sm);
 }
@@ -477,6 +484,214 @@ __device__ __forceinline__ void RowPass<T, NX2, RB, R, BX, G, PP>::run(const Ins
 
   double sums[3] = {esum, gxs, gys};
   block_sum<3, NT>(sums, red);
+  if (t == 0) {
+    double* p = d.partials + (size_t)tile * 4;
+    p[0] = sums[0];
+    p[1] = sums[1];
+    p[2] = sums[2];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row pass with paired rows (option bit 32).  Rows a = yy and b = yy + RB/2
+// of the block's brick row travel together as z = a + i b through one
+// length-N (N = Nx) complex transform, so the real-FFT even/odd split and its
+// twiddles are not needed:
+//   C2R: Z[k] = A[k] + i B[k] and Z[N-k] = conj A[k] + i conj B[k] for
+//        0 < k < N/2, Z[0] = Re A[0] + i Re B[0], Z[N/2] = 0; z = IFFT_N(Z)
+//        gives a in Re z, b in Im z (the same unnormalised inverse as the
+//        half-length path);
+//   R2C: Z = FFT_N(p_a + i p_b), 2A[k] = Z[k] + conj Z[N-k],
+//        2B[k] = -i (Z[k] - conj Z[N-k]) for k < N/2 (twice the half
+//        spectra, as the half-length path stores them).
+// Lane t -> pair yy = t % (RB/2), transform lane j = t / (RB/2); a lane holds
+// positions j + r*S (S = N/R) as in fft_multi: r < R/2 are the half-spectrum
+// columns it loads and stores, r >= R/2 their mirror images, exchanged
+// through LDS.  Products, partial sums and the outputs are those of RowPass.
+template <class T, int NX2, int RB, int R, int BX, int G, int PP>
+struct RowPassPaired {
+  using Real = T;
+  static constexpr int N = 2 * NX2;         // full row length
+  static constexpr int W2 = RB / 2;         // row pairs of the block
+  static constexpr int S = N / R;           // lanes per pair
+  static constexpr int NT = W2 * S;         // = RB * NX2 / R, the block of RowPass
+  static_assert(RB % 2 == 0 && R % 2 == 0, "pairs of rows, half of each lane's points mirrored");
+  using View = LdsView<T, N, W2, R, G, PP & 1>;
+  struct alignas(16) Smem {
+    cx<T> lds_mem[View::kElems];
+    cx<T> tw[TwLayout<N, R>::kAlloc];
+    double red[4 * ((NT + 63) / 64)];
+  };
+  __device__ static void run(const InstDesc<T>* __restrict__ descs, int tiles_per_inst,
+                             const cx<T>* __restrict__ pool, int block, Smem& sm);
+};
+
+template <class T, int NX2, int RB, int R, int BX, int G, int PP>
+__device__ __forceinline__ void RowPassPaired<T, NX2, RB, R, BX, G, PP>::run(const InstDesc<T>* __restrict__ descs,
+                                                                            int tiles_per_inst,
+                                                                            const cx<T>* __restrict__ pool, int block,
+                                                                            Smem& sm) {
+  static_assert(G == 1 || G == 3, "group of 1 or 3 fields");
+  static_assert(S % BX == 0, "pair lanes cover whole bricks");
+  constexpr int GR = G == 3 ? 2 : 1;   // R2C group
+  constexpr int H = R / 2;             // loaded / stored points per lane and row
+  constexpr int kCP = (PP & 4) ? kCacheNt : kCacheSc1;
+  const int inst = block / tiles_per_inst;
+  const int tile = block - inst * tiles_per_inst;
+  const InstDesc<T>& d = descs[inst];
+  if (d.nx == 0) return;   // an empty entry of an engine's descriptor array: the whole block exits
+  const int t = threadIdx.x;
+  const int yy = t % W2;   // pair: rows yy and yy + W2
+  const int j = t / W2;
+  View lds{sm.lds_mem, yy};
+
+  // brick offsets as in RowPass; row b sits W2 rows below row a in every brick
+  const unsigned tile_base = (unsigned)tile * (NX2 / BX) * (RB * BX) + yy * BX;
+  auto addr = [&](int k) -> unsigned { return tile_base + (unsigned)(k / BX) * (RB * BX) + (k % BX); };
+  constexpr unsigned ESZ = sizeof(cx<T>);
+  const unsigned va = addr(j) * ESZ, vbo = (unsigned)(W2 * BX) * ESZ;
+  auto rb = [](int r) -> unsigned { return (unsigned)r * S * RB * ESZ; };   // column j + r*S
+
+  // f[field][r]: r < H: A at column j + r*S; r >= H: B at column j + (r-H)*S
+  cx<T> f3[3][R];
+  {
+    const BufField<cx<T>> gum(d.um), gvm(d.vm), gwm(d.wm);
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+      f3[0][r] = gum.template ld<kCacheNt>(va, rb(r));
+      f3[1][r] = gvm.template ld<kCacheNt>(va, rb(r));
+      f3[2][r] = gwm.template ld<kCacheNt>(va, rb(r));
+      f3[0][r + H] = gum.template ld<kCacheNt>(va + vbo, rb(r));
+      f3[1][r + H] = gvm.template ld<kCacheNt>(va + vbo, rb(r));
+      f3[2][r + H] = gwm.template ld<kCacheNt>(va + vbo, rb(r));
+    }
+  }
+  stage_fft_twiddles<T, NT, N, R>(sm.tw, tw_of(pool, N));
+
+  // C2R assembly of one field: A, B of column k stored at k and N/2 + k; the
+  // lane's mirrored positions p = j + r*S (r >= H, p > N/2) read column N - p.
+  auto c2r_store = [&](cx<T>(&q)[R], int f) {
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+      lds.store_plain(f, j + r * S, q[r]);
+      lds.store_plain(f, NX2 + j + r * S, q[r + H]);
+    }
+  };
+  auto c2r_assemble = [&](cx<T>(&q)[R], int f) {
+    cx<T> z[R];
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+      cx<T> a = q[r], b = q[r + H];
+      if (r == 0 && j == 0) { a.im = (T)0; b.im = (T)0; }   // column 0: imaginary parts ignored
+      z[r] = cx<T>{a.re - b.im, a.im + b.re};
+    }
+#pragma unroll
+    for (int r = H; r < R; ++r) {
+      const int p = j + r * S;   // > N/2 except p == N/2 (j == 0, r == H): the Nyquist column, zero
+      if (r == H && j == 0) {
+        z[r] = cx<T>{(T)0, (T)0};
+      } else {
+        const cx<T> a = lds.load_plain(f, N - p), b = lds.load_plain(f, NX2 + N - p);
+        z[r] = cx<T>{a.re + b.im, b.re - a.im};
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) q[r] = z[r];
+  };
+  if constexpr (G == 3) {
+#pragma unroll
+    for (int f = 0; f < 3; ++f) c2r_store(f3[f], f);
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < 3; ++f) c2r_assemble(f3[f], f);
+    __syncthreads();
+    fft_multi<N, R, +1, 3>(f3, j, lds, sm.tw);
+  } else {
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      cx<T>(&q)[1][R] = *reinterpret_cast<cx<T>(*)[1][R]>(&f3[f]);
+      __syncthreads();
+      c2r_store(q[0], 0);
+      __syncthreads();
+      c2r_assemble(q[0], 0);
+      __syncthreads();
+      fft_multi<N, R, +1, 1>(q, j, lds, sm.tw);
+    }
+  }
+
+  // products: (re, im) of each point are (row a, row b) at one x
+  const T lam = (T)d.lam, bet = (T)d.beta;
+  T esum_l = 0, gxs_l = 0, gys_l = 0;
+  cx<T> f2[2][R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    T o1[2], o2[2];
+    const T uxa[2] = {f3[0][r].re, f3[0][r].im}, uya[2] = {f3[1][r].re, f3[1][r].im};
+    const T wa[2] = {f3[2][r].re, f3[2][r].im};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const T ux = uxa[h], uy = uya[h], wv = wa[h];
+      const T A = wv * ux, Wp = wv * uy;
+      const T Q = ux * ux + uy * uy;
+      const T Cx = Q * ux, Cy = Q * uy;
+      o1[h] = lam * A + bet * Cy;
+      o2[h] = lam * Wp - bet * Cx;
+      esum_l += Q;
+      gxs_l += Cx;
+      gys_l += Cy;
+    }
+    f2[0][r] = cx<T>{o1[0], o1[1]};
+    f2[1][r] = cx<T>{o2[0], o2[1]};
+  }
+  const double esum = (double)esum_l, gxs = (double)gxs_l, gys = (double)gys_l;
+
+  // R2C: forward transforms, then the mirror exchange (the lane's r < H
+  // columns k read Z[N - k]; column 0 reads itself)
+  const T tkx = (T)d.tkx;
+  const BufField<cx<T>> gpm(d.pm), gqm(d.qm);
+  auto r2c_out = [&](cx<T>(&q)[R], int f, const BufField<cx<T>>& g, bool is_p) {
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+      const int k = j + r * S;
+      const cx<T> zk = q[r], zm = cconj(lds.load_plain(f, (N - k) & (N - 1)));
+      cx<T> a2 = zk + zm;                 // 2A[k]
+      cx<T> b2 = cmuli(zm - zk);          // 2B[k] = -i (Z[k] - conj Z[N-k])
+      if (is_p) {                         // P = -i kx (twice the half spectrum)
+        const T kx = tkx * (T)k;
+        a2 = cx<T>{kx * a2.im, -kx * a2.re};
+        b2 = cx<T>{kx * b2.im, -kx * b2.re};
+      }
+      g.template st<kCP>(va, rb(r), a2);
+      g.template st<kCP>(va + vbo, rb(r), b2);
+    }
+  };
+  if constexpr (GR == 2) {
+    __syncthreads();
+    fft_multi<N, R, -1, 2>(f2, j, lds, sm.tw);
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds.store_plain(f, j + r * S, f2[f][r]);
+    __syncthreads();
+    r2c_out(f2[0], 0, gpm, true);
+    r2c_out(f2[1], 1, gqm, false);
+  } else {
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      cx<T>(&q)[1][R] = *reinterpret_cast<cx<T>(*)[1][R]>(&f2[f]);
+      __syncthreads();
+      fft_multi<N, R, -1, 1>(q, j, lds, sm.tw);
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds.store_plain(0, j + r * S, q[0][r]);
+      __syncthreads();
+      r2c_out(q[0], 0, f == 0 ? gpm : gqm, f == 0);
+    }
+  }
+
+  double sums[3] = {esum, gxs, gys};
+  block_sum<3, NT>(sums, sm.red);
   if (t == 0) {
     double* p = d.partials + (size_t)tile * 4;
     p[0] = sums[0];

@@ -87,13 +87,16 @@ def test_tuned_tables_name_instantiated_schedules(af):
 
 def test_schedule_struct_fields(af):
     """af_schedule carries the column-block width (divides the brick width)
-    and the option bits (LDS double buffering; streaming stores, FP32 only)."""
+    and the option bits (LDS double buffering; streaming stores, FP32 only;
+    paired rows, with an even brick height)."""
     for prec in (32, 64):
         for sc in af.schedules(prec, 1024):
             by, bx, rr, rg, cr, gf, gi, cw, opts = sc
             assert bx % cw == 0 and rr in (4, 8, 16) and cr in (4, 8, 16)
-            assert 0 <= opts < 32 and (prec == 32 or opts in (0, 1))
+            assert 0 <= opts < 64 and (prec == 32 or opts & 31 in (0, 1))
+            assert not opts & 32 or by % 2 == 0
     assert any(sc[8] & 6 for sc in af.schedules(32, 1024))
+    assert any(sc[8] & 32 for sc in af.schedules(32, 1024)) and any(sc[8] & 32 for sc in af.schedules(64, 1024))
 
 
 def test_autotune_merge_keeps_untuned_lines():
