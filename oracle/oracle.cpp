@@ -13,7 +13,12 @@
 //                            cubic closed form, energy closed form
 //   Stepper::step            lambda=beta=0 exact E^n, mean-ODE scalar
 //                            recurrence, uniform-advection O(dt^2) closed form,
-//                            measured temporal order 2 (variant IF-RK)
+//                            measured temporal order 2 (variant IF-RK); mean
+//                            ODE with sigma(0)=0 and omega=0 converging at
+//                            order 4 to |M(t)|^2 = |M0|^2/(1+2b|M0|^2 t/N^4)
+//   Stepper::project         lambda=beta=0 one step from a kx=0 column with
+//                            independent rows y, Ny-y equals P(E omega0)
+//                            exactly (average-and-conjugate, Nyquist row kept)
 //   masks                    brute-force lattice counts
 #include "oracle.h"
 
