@@ -163,7 +163,7 @@ def roofline(af, r):
 
 
 def ensemble_leg(af, steps=200, seed=7, measured_sequential=False):
-    """Config 4: 64 instances (32 x 256^2, 24 x 512^2, 8 x 1024^2), seeded random
+    """The mixed ensemble: 64 instances (32 x 256^2, 24 x 512^2, 8 x 1024^2), seeded random
     alpha, beta, lambda0, FP32, stepped together in shared launches; compared
     with the same instances run one after another through the single-run path
     (timed per size class and summed, or with measured_sequential every one
@@ -250,8 +250,8 @@ def ensemble_leg(af, steps=200, seed=7, measured_sequential=False):
             / HBM_PEAK_GBS}
 
 
-def config3_leg(af, steps=5000):
-    """Config 3 at its full length: single FP32 runs of 2048^2 and 4096^2 for
+def large_grid_leg(af, steps=5000):
+    """The large single runs at full length: FP32 runs of 2048^2 and 4096^2 for
     lambda0 in {2, 3.5, 5} (alpha -0.8, beta 0.01, Gamma0 -0.045, Gamma2 2.5e-5),
     dt 5e-5, `steps` steps with per-step energy from the seeded band-limited
     IC; one line per run."""
@@ -270,12 +270,12 @@ def config3_leg(af, steps=5000):
             out.append({"grid": [n, n], "lambda0": lam, "steps": steps, "value": n * n * steps / wall,
                         "ms_per_step": wall / steps * 1e3, "energy_first": float(e[0]), "energy_last": float(e[-1]),
                         "finite": bool(np.all(np.isfinite(e))), "fp32_error": err})
-            print(json.dumps({"config3": out[-1]}), flush=True)
+            print(json.dumps({"large_grid": out[-1]}), flush=True)
     return out
 
 
 def batched_leg(af, n, count, steps, seed=3):
-    """Config 3 sizes stepped as a batch: `count` FP32 instances of n x n with
+    """Large grids stepped as a batch: `count` FP32 instances of n x n with
     lambda0 cycling over {2, 3.5, 5} (other parameters as the headline), in
     shared launches of the ensemble engine (its tuned batched schedule)."""
     eng = af.Engine(200 << 30)
@@ -300,7 +300,7 @@ def batched_leg(af, n, count, steps, seed=3):
 
 
 def serving_leg(af, rate, requests=16, seed=11, horizon_scale=1.0):
-    """Config 5: Poisson arrivals at `rate`/s over a window of requests/rate s
+    """The served workload: Poisson arrivals at `rate`/s over a window of requests/rate s
     (about `requests` arrivals); N from {128..2048}^2 with weights
     0.3/0.3/0.2/0.15/0.05; horizon U{1k..50k} steps (times horizon_scale);
     snapshot every 500 steps; alpha, beta, lambda0 as the ensemble; FP32;
@@ -348,17 +348,17 @@ def serving_leg(af, rate, requests=16, seed=11, horizon_scale=1.0):
     return out
 
 
-def cpu_configs():
-    """The oracle per workload of the configuration list, on all host cores
-    (up to 16) and on one thread: median time of single steps (energy on)
+def cpu_workloads():
+    """The oracle on the 256^2 FP64, 1024^2 FP64 / FP32 and 2048^2 FP32
+    workloads, on all host cores (up to 16) and on one thread: median time of single steps (energy on)
     after warm-up steps (10 on all cores, 2 on one thread), one JSON line per
     leg.  Steps on one thread are fewer to bound the run."""
     from oracle.oracle_py import Oracle
     af = importlib.import_module("2d_active_matter_simulation_amd")
     threads = min(os.cpu_count() or 1, 16)
-    legs = [("config 1: 256^2 FP64, dt 2e-4, all 2000 steps", 256, 64, 2e-4, 2000, 200),
-            ("config 2: 1024^2 FP64, dt 1e-4, first 200 steps", 1024, 64, 1e-4, 200, 5),
-            ("config 2: 1024^2 FP32, dt 1e-4, first 200 steps", 1024, 32, 1e-4, 200, 5),
+    legs = [("256^2 FP64, dt 2e-4, all 2000 steps", 256, 64, 2e-4, 2000, 200),
+            ("1024^2 FP64, dt 1e-4, first 200 steps", 1024, 64, 1e-4, 200, 5),
+            ("1024^2 FP32, dt 1e-4, first 200 steps", 1024, 32, 1e-4, 200, 5),
             ("2048^2 FP32, dt 5e-5, 10 steps", 2048, 32, 5e-5, 10, 2)]
     out = []
     for name, n, prec, dt, steps, steps1 in legs:
@@ -446,20 +446,20 @@ def main():
     ap.add_argument("--ensemble-full", type=int, default=0,
                     help="only run the 64-instance ensemble for this many steps and compare it with each "
                          "of the 64 runs stepped alone (measured), printing one JSON line")
-    ap.add_argument("--config3", type=int, default=0,
+    ap.add_argument("--large-grids", type=int, default=0,
                     help="only run single FP32 runs of 2048^2 and 4096^2 for lambda0 in {2, 3.5, 5} for this "
                          "many steps (from the first step, energy on), one JSON line per run")
-    ap.add_argument("--cpu-configs", action="store_true",
-                    help="only time the CPU oracle on the configuration workloads (all host cores and one "
+    ap.add_argument("--cpu-workloads", action="store_true",
+                    help="only time the CPU oracle on the 256^2, 1024^2 and 2048^2 workloads (all host cores and one "
                          "thread), one JSON line per leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(instances would shard across GPUs without communication).", file=sys.stderr)
         sys.exit(2)
-    if args.cpu_configs:
+    if args.cpu_workloads:
         print(json.dumps({"host": {"cpu_count": os.cpu_count(), "model": cpu_model()}}), flush=True)
-        cpu_configs()
+        cpu_workloads()
         return
     if args.copy_calibration:
         import torch
@@ -475,8 +475,8 @@ def main():
     af.lib()
 
     N = args.grid
-    if args.config3 > 0:
-        config3_leg(af, args.config3)
+    if args.large_grids > 0:
+        large_grid_leg(af, args.large_grids)
         return
     if args.ensemble_full > 0:
         print(json.dumps({"ensemble": ensemble_leg(af, steps=args.ensemble_full, measured_sequential=True)}))

@@ -39,7 +39,7 @@ def _as64(af, p):
     return q
 
 
-def test_config4_ensemble_against_oracle(af, orc):
+def test_mixed_ensemble_64_runs_against_oracle(af, orc):
     rng = np.random.default_rng(7)
     mix = [(256, 32), (512, 24), (1024, 8)]
     eng = af.Engine(200 << 30)
@@ -80,7 +80,7 @@ def test_config4_ensemble_against_oracle(af, orc):
     eng.close()
 
 
-def test_config5_served_mix_against_oracle(af, orc):
+def test_served_mix_128_to_2048_against_oracle(af, orc):
     rng = np.random.default_rng(5)
     reqs = []
     t = 0.0

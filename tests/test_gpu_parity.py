@@ -46,15 +46,15 @@ def _as64(af, p):
 
 
 @pytest.mark.parametrize("seed", [1, 2])
-def test_parity_fp64_config1(af, orc, seed):
+def test_parity_fp64_256(af, orc, seed):
     """256^2, dt = 2e-4, band-limited IC (amp 1e-3, |k| <= 10), 100 steps."""
     p = af.params(256, dt=2e-4, precision=64, **JW)
     w0 = af.ic_bandlimited(256, seed=seed, amp=1e-3, kmax=10.0)
     compare(af, orc, p, w0, (0.0, 0.0), (1, 9, 90), 1e-12, 1e-12)
 
 
-def test_parity_fp64_config1_energy_series(af, orc):
-    """256^2 FP64 config in full: the per-step energy series of 2000 steps and
+def test_parity_fp64_256_energy_series(af, orc):
+    """256^2 FP64 at its full length: the per-step energy series of 2000 steps and
     the final state match the oracle (the flow is in its linear growth phase,
     where rounding differences do not grow relative to the field)."""
     p = af.params(256, dt=2e-4, precision=64, **JW)

@@ -63,7 +63,7 @@ def _averages(samples):
 
 
 def test_long_run_fp32_vs_fp64_statistics(af):
-    """Config 2 window (steps 10k-20k) is reported; the flow is still evolving
+    """1024^2, dt 1e-4: the window of steps 10k-20k is reported; the flow is still evolving
     there, so the agreement is asserted within the batch-means error bars.  The
     run continues to 60k steps and the window 20k-60k is compared the same way
     (energy difference also reported against the 1% target)."""
