@@ -12,7 +12,11 @@
 //               agent-scope release (L2 write-back) before it arrives and an
 //               agent-scope acquire (L2 invalidate) after it leaves.
 //               Mode 1: every lane fences; mode 2: the block's lanes meet at
-//               a workgroup barrier and lane 0 fences and arrives.
+//               a workgroup barrier and lane 0 fences and arrives; mode 3:
+//               arrivals are counted per XCD (the XCC_ID hardware register),
+//               and only the last workgroup of an XCD to arrive writes that
+//               XCD's L2 back and arrives at the global count, after every
+//               workgroup of the XCD waited for its own stores to reach L2.
 //
 // Each phase every block writes NV float4 per lane into its slice of one of
 // two buffers; after the boundary each block reads the slice another block
@@ -97,14 +101,66 @@ __device__ __forceinline__ void grid_barrier(unsigned* count, unsigned target, u
   if (MODE == 1) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
+// Waits until *c >= target or the abort flag is up (raising it past the spin
+// limit); lane 0 only.
+__device__ __forceinline__ void spin_until(unsigned* c, unsigned target, unsigned* abort_flag) {
+  unsigned spins = 0;
+  while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    if (++spins > kSpinLimit) {
+      __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+__device__ __forceinline__ unsigned xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15u; }
+
+// Mode 3 (see the header).  xcd_count[16] counts arrivals per XCD (monotonic),
+// pop[x] is the number of workgroups on XCD x and nxcd the number of XCDs used.
+__device__ __forceinline__ void grid_barrier_xcd(unsigned* count, unsigned* xcd_count, unsigned pop, unsigned nxcd,
+                                                 unsigned p, unsigned* abort_flag) {
+  __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wave's stores have reached the L2
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned old = __hip_atomic_fetch_add(&xcd_count[xcc_id()], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old + 1 == (p + 1) * pop) {   // last of its XCD: every store of the XCD is in its L2
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    spin_until(count, (p + 1) * nxcd, abort_flag);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
 template <int NV, int MODE, bool NT>
 __global__ void __launch_bounds__(kLanes) persistent(float4* b0, float4* b1, int phases, unsigned* count,
                                                      unsigned* abort_flag, unsigned* err) {
   const int b = blockIdx.x, nb = gridDim.x;
+  // mode 3: count + 1 holds the start count, count + 16.. the per-XCD
+  // populations and count + 32.. the per-XCD arrival counts
+  __shared__ unsigned pop_nx[2];
+  if constexpr (MODE == 3) {
+    if (threadIdx.x == 0) {
+      __hip_atomic_fetch_add(&count[16 + xcc_id()], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&count[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      spin_until(&count[1], (unsigned)nb, abort_flag);
+      unsigned nx = 0;
+      for (int x = 0; x < 16; ++x) nx += __hip_atomic_load(&count[16 + x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0;
+      pop_nx[0] = __hip_atomic_load(&count[16 + xcc_id()], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      pop_nx[1] = nx;
+    }
+    __syncthreads();
+  }
   for (int p = 0; p < phases; ++p) {
     float4* buf = (p & 1) ? b1 : b0;
     write_phase<NV, NT>(buf, p, b);
-    grid_barrier<MODE>(count, (unsigned)(p + 1) * nb, abort_flag);
+    if constexpr (MODE == 3)
+      grid_barrier_xcd(count, count + 32, pop_nx[0], pop_nx[1], (unsigned)p, abort_flag);
+    else
+      grid_barrier<MODE>(count, (unsigned)(p + 1) * nb, abort_flag);
     check_phase<NV>(buf, p, b, nb, err);
   }
 }
@@ -162,7 +218,7 @@ void run_persistent(const Bufs& m, int phases, hipStream_t s) {
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
   for (int rep = 0; rep < 2; ++rep) {
-    CK(hipMemset(m.count, 0, 4));
+    CK(hipMemset(m.count, 0, 48 * sizeof(unsigned)));
     CK(hipMemset(m.abort_flag, 0, 4));
     CK(hipMemset(m.err, 0, 4));
     CK(hipDeviceSynchronize());
@@ -208,7 +264,7 @@ int main(int argc, char** argv) {
   const size_t n = (size_t)kBlocks * kLanes * kMaxNv;
   CK(hipMalloc(&m.b0, n * sizeof(float4)));
   CK(hipMalloc(&m.b1, n * sizeof(float4)));
-  CK(hipMalloc(&m.count, 4));
+  CK(hipMalloc(&m.count, 48 * sizeof(unsigned)));
   CK(hipMalloc(&m.abort_flag, 4));
   CK(hipMalloc(&m.err, 4));
   hipStream_t s;
@@ -221,6 +277,8 @@ int main(int argc, char** argv) {
   run_persistent<8, 1, false>(m, phases, s);
   run_persistent<8, 2, false>(m, phases, s);
   run_persistent<8, 2, true>(m, phases, s);
+  run_persistent<1, 3, false>(m, phases, s);
+  run_persistent<8, 3, false>(m, phases, s);
   CK(hipStreamDestroy(s));
   CK(hipFree(m.b0));
   CK(hipFree(m.b1));
