@@ -152,10 +152,16 @@ def run_workload(af, n, prec, steps, warmup, seed=1, host_io=False, device_warmu
                 prof_ms_per_step=tot_ms / (5 * kprof), host_io=io, fp32_error=err)
 
 
-def roofline(af, r):
+def pass_bytes_ms(af, r):
+    """Algorithmic bytes per launch and mean event-timed launch duration of the
+    two passes of run r."""
     fb = field_bytes(r["n"], r["prec"])
-    kernels = {"col_pass": (col_fields(af, r["n"], r["prec"]) * fb, r["col_ms"]),
-               "row_pass": (ROW_FIELDS * fb, r["row_ms"])}
+    return {"col_pass": (col_fields(af, r["n"], r["prec"]) * fb, r["col_ms"]),
+            "row_pass": (ROW_FIELDS * fb, r["row_ms"])}
+
+
+def roofline(af, r):
+    kernels = pass_bytes_ms(af, r)
     name = max(kernels, key=lambda k: kernels[k][1])
     alg, ms = kernels[name]
     achieved = alg / (ms * 1e-3) / 1e9
@@ -522,6 +528,14 @@ def main():
     tr_bytes = None
     if traffic and f"{name}_bytes_per_launch" in traffic:
         tr_bytes = traffic[f"{name}_bytes_per_launch"]
+    # both passes of the headline run, each against the HBM peak (the dominant
+    # one is the roofline entry itself)
+    per_pass = {}
+    for k, (b, ms) in pass_bytes_ms(af, head).items():
+        gbs = b / (ms * 1e-3) / 1e9
+        per_pass[k] = {"achieved": gbs, "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": b,
+                       "avg_launch_ms": ms,
+                       "traffic": traffic.get(f"{k}_bytes_per_launch") if traffic else None}
     if not args.no_extra and N == 1024:
         extra["fp32_over_fp64_1024"] = head["value"] / extra["1024_fp64"]["value"]
     out = {
@@ -547,7 +561,8 @@ def main():
                      "algorithmic_bytes_per_launch": alg, "avg_launch_ms": kms,
                      "events_over_steps": head["prof_steps"],
                      "events_ms_per_step": head["prof_ms_per_step"],
-                     "step_model_frac": step_model_bytes(N) / (head["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                     "step_model_frac": step_model_bytes(N) / (head["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "passes": per_pass},
         "fp32_error": head["fp32_error"],
         "extra": extra,
     }
