@@ -1,6 +1,9 @@
-"""Every instantiated schedule (brick shape, radix, LDS grouping) of a size
-computes the same step: FP64 against the oracle to 1e-12 after 20 steps,
-FP32 against the FP64 oracle to 2e-6."""
+"""Every instantiated schedule (brick shape, radix, LDS grouping, options) of a
+size computes the same step: at 256^2 and 512^2 FP64 against the oracle to
+1e-12 after 20 steps and FP32 against the FP64 oracle to 2e-6; at 1024^2,
+2048^2 and 4096^2 the same bounds after one step."""
+import os
+
 import numpy as np
 import pytest
 
@@ -36,6 +39,41 @@ def test_every_schedule_matches_oracle(af, orc, prec, n):
             assert np.allclose(m, mo, rtol=10 * tol), (i, sc)
     finally:
         af.select_schedule(prec, n, n, -1)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("n", [1024, 2048, 4096])
+def test_every_schedule_one_step_at_large_sizes(af, orc, prec, n):
+    """Every instantiated schedule of the sizes the tuner chooses among for
+    the benchmarks (radix 4/8/16, grouped or single-field exchanges, brick
+    shapes, cache options, paired rows): one step from an O(1) band-limited
+    state with a mean flow against one FP64 oracle step, FP64 to 1e-12 and
+    FP32 to 2e-6 (relative L2 on omega); energy and mean velocity alike."""
+    p = af.params(n, dt=1e-4 if n <= 1024 else 5e-5, precision=prec, **JW)
+    w0 = af.ic_bandlimited(n, seed=11, amp=2.0, kmax=12.0)
+    M0 = (0.05 * n * n, -0.02 * n * n)
+    q = af.AfParams.from_buffer_copy(p)
+    q.precision = 64
+    o = orc.Oracle.from_params(q, nthreads=min(16, os.cpu_count() or 1))
+    o.set_state(w0, M0)
+    eo = o.step(1, energy=True)
+    wo, mo = o.get_state()
+    tol = 1e-12 if prec == 64 else 2e-6
+    scheds = af.schedules(prec, n)
+    try:
+        for i, sc in enumerate(scheds):
+            af.select_schedule(prec, n, n, i)
+            run = af.Run(p)
+            run.set_state(w0, M0)
+            e = run.step(1, energy=True)
+            w, m = run.get_state()
+            run.close()
+            assert rel_l2(w, wo) <= tol, (i, sc, rel_l2(w, wo))
+            assert abs(e[0] - eo[0]) <= tol * abs(eo[0]), (i, sc)
+            assert np.allclose(m, mo, rtol=10 * tol), (i, sc)
+    finally:
+        af.select_schedule(prec, n, n, -1)
+    print(f"\n{len(scheds)} schedules of {n}^2 FP{prec} match the oracle")
 
 
 def test_narrow_blocks_reduce_every_partial(af, orc):
