@@ -5,6 +5,8 @@ Tolerances: FP64 <= 1e-12 relative L2 on omega (real-space norm via Parseval
 weights) after 100 steps; FP32 (FP32 fields and FFTs, FP64 integrating
 factors and reductions) against the FP64 oracle <= 2e-5 after 100 steps
 (measured ~5e-6)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -80,6 +82,33 @@ def test_parity_fp64_reference_operator(af, orc, nx, ny, dealias, mean_scheme):
                   precision=64, dealias=dealias, mean_scheme=mean_scheme)
     w0 = af.ic_bandlimited(nx, ny, seed=11, amp=0.5, kmax=8.0)
     compare(af, orc, p, w0, (0.3 * nx * ny, 0.1 * nx * ny), (3, 47), 1e-12, 1e-12)
+
+
+def test_every_grid_shape_one_step(af, orc):
+    """Every supported grid, Nx and Ny powers of two from 32 to 4096 (64
+    shapes, each its own pair of row and column kernels with the default or
+    tuned schedule), one step in FP64 and in FP32 from the same O(1) state with
+    a mean flow, against one FP64 oracle step: FP64 to 1e-12, FP32 to 2e-6."""
+    sizes = [32, 64, 128, 256, 512, 1024, 2048, 4096]
+    nthreads = min(16, os.cpu_count() or 1)
+    for nx in sizes:
+        for ny in sizes:
+            dt = 1e-4 if max(nx, ny) <= 1024 else 5e-5
+            w0 = af.ic_bandlimited(nx, ny, seed=13, amp=2.0, kmax=10.0)
+            M0 = (0.05 * nx * ny, -0.02 * nx * ny)
+            o = oracle_for(orc, af.params(nx, ny, dt=dt, precision=64, **JW), nthreads)
+            o.set_state(w0, M0)
+            eo = o.step(1, energy=True)
+            wo, mo = o.get_state()
+            for prec, tol in ((64, 1e-12), (32, 2e-6)):
+                run = af.Run(af.params(nx, ny, dt=dt, precision=prec, **JW))
+                run.set_state(w0, M0)
+                e = run.step(1, energy=True)
+                w, m = run.get_state()
+                run.close()
+                assert rel_l2(w, wo) <= tol, (nx, ny, prec, rel_l2(w, wo))
+                assert abs(e[0] - eo[0]) <= tol * abs(eo[0]), (nx, ny, prec)
+                assert np.allclose(m, mo, rtol=10 * tol), (nx, ny, prec)
 
 
 def test_parity_fp32_error_stated(af, orc):
