@@ -415,7 +415,7 @@ static bool capture_steps(af_run* r, int steps, hipGraph_t* graph, hipGraphExec_
   return true;
 }
 
-// Captures, instantiates and uploads both step graphs (once per run: they
+// Captures, instantiates and uploads the step graphs of 1, 2, 4, 8 and 16 steps (once per run: they
 // depend only on the run's buffers, not on its state), so that the first
 // chunk of steps after set_state already replays resident graphs.  A failed
 // capture is not an error: the run then launches its kernels directly.
