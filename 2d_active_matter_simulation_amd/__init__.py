@@ -380,17 +380,21 @@ class Request:
 def served_dt(n, lam, alpha, beta):
     """Time step for a request of the served workload (N^2 grid, box 2pi):
     1e-4 up to 1024^2 and 5e-5 at 2048^2, lowered further by an advective
-    CFL estimate dt <= 4.5 / (lam u_max k_max), k_max = N/3 (2/3 mask) and
+    CFL estimate dt <= 2 / (lam u_max k_max), k_max = N/3 (2/3 mask) and
     u_max = 3 sqrt(-2 alpha / beta) (three times the saturated rms velocity,
     sqrt(-2 alpha/beta), which the 1024^2 runs with alpha = -0.8, beta = 0.01
-    show: <E> ~ 81).  The coefficient 4.5 is the value of 1024^2 at dt = 1e-4,
-    lam = 3.5, which is stable over 300k steps; 256^2 requests at dt = 2e-4
-    with lam in [4, 5] went NaN in the served workload."""
+    show: <E> ~ 81).  The coefficient 2 keeps the advective term inside the
+    explicit RK4 stability interval on the imaginary axis (|z| <= 2.8) with a
+    margin.  The 1024^2 run at dt = 1e-4, lam = 3.5 sits at 4.5 by this
+    measure and is stable over 300k steps, but a 512^2 request at 2.9
+    (lam 4.17, beta 0.0077, dt 1e-4) went NaN after 18.5k steps in FP32 while
+    its FP64 run stayed finite: near the limit, whether a chaotic run survives
+    depends on its rounding."""
     base = 1e-4 if n <= 1024 else 5e-5
     if lam <= 0 or alpha >= 0 or beta <= 0:
         return base
     umax = 3.0 * np.sqrt(-2.0 * alpha / beta)
-    return float(min(base, 4.5 / (lam * umax * (n / 3.0))))
+    return float(min(base, 2.0 / (lam * umax * (n / 3.0))))
 
 
 def serve(requests, hbm_budget, max_tick_steps=50, on_snapshot=None, keep_final=False):
