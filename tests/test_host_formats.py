@@ -237,3 +237,8 @@ def test_cli_failure_injection(tmp_path):
     r = run_cli([str(d)], tmp_path)
     assert r.returncode == 1 and "Error: unable to create snapshot file" in r.stderr
     assert "initializing the time stepping" not in r.stdout
+    # a grid size outside the supported powers of two (the reference's cuFFT takes any size): refused
+    # with a message when the run is created, before any device call
+    (d / "specifications.txt").write_text("T:\n1\ndt:\n0.1\nNx:\n96\nNy:\n64\ninitial_condition:\nblank\n")
+    r = run_cli([str(d)], tmp_path)
+    assert r.returncode == 1 and "powers of two in [32, 4096]" in r.stderr, r.stderr
