@@ -55,11 +55,13 @@ PAIRED_BASES = {
 }
 # FP32 schedules that also get the cache-policy variants (options | 2 ... 14,
 # and | 16 ... 30 for radix-16 column passes): the fastest single-run and
-# batched schedules of each size.
+# batched schedules of each size, and the engine's choices for the mixed
+# ensemble (4 x 4 bricks, 4-column blocks).
 STORE_VARIANT_BASES = {
-    256: [(4, 2, 4, 3, 4, 2, 3, 2, 1)],
-    512: [(4, 2, 4, 3, 4, 2, 3, 1, 0), (4, 4, 8, 3, 8, 2, 3, 4, 0)],
-    1024: [(4, 2, 4, 3, 4, 2, 3, 2, 1), (4, 2, 8, 1, 8, 1, 1, 2, 0)],
+    256: [(4, 2, 4, 3, 4, 2, 3, 2, 1), (4, 4, 8, 1, 16, 1, 1, 4, 0)],
+    512: [(4, 2, 4, 3, 4, 2, 3, 1, 0), (4, 4, 8, 3, 8, 2, 3, 4, 0), (4, 4, 8, 1, 8, 1, 1, 4, 0),
+          (4, 4, 8, 1, 16, 1, 1, 4, 0)],
+    1024: [(4, 2, 4, 3, 4, 2, 3, 2, 1), (4, 2, 8, 1, 8, 1, 1, 2, 0), (4, 4, 8, 1, 16, 1, 1, 4, 0)],
     2048: [(4, 2, 16, 1, 16, 1, 1, 2, 0), (4, 2, 16, 1, 16, 1, 1, 2, 1), (4, 4, 8, 1, 16, 1, 1, 4, 0)],
     4096: [(4, 4, 16, 1, 16, 1, 1, 2, 0)],
 }
