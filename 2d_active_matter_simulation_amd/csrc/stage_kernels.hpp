@@ -336,15 +336,18 @@ __device__ __forceinline__ void RowPass<T, NX2, RB, R, BX, G, PP>::run(const Ins
   constexpr unsigned ESZ = sizeof(cx<T>);
   const unsigned vb = addr(j) * ESZ;
   auto rb = [](int r) -> unsigned { return (unsigned)r * TR * RB * ESZ; };
+  // Field by field: loads complete in issue order, so the first field's
+  // exchange (and with G = 1 its whole transform) waits for its own loads
+  // only while the other two fields are still in flight.
   cx<T> f3[3][R];
   {
     const BufField<cx<T>> gum(d.um), gvm(d.vm), gwm(d.wm);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      f3[0][r] = gum.template ld<kCacheNt>(vb, rb(r));
-      f3[1][r] = gvm.template ld<kCacheNt>(vb, rb(r));
-      f3[2][r] = gwm.template ld<kCacheNt>(vb, rb(r));
-    }
+    for (int r = 0; r < R; ++r) f3[0][r] = gum.template ld<kCacheNt>(vb, rb(r));
+#pragma unroll
+    for (int r = 0; r < R; ++r) f3[1][r] = gvm.template ld<kCacheNt>(vb, rb(r));
+#pragma unroll
+    for (int r = 0; r < R; ++r) f3[2][r] = gwm.template ld<kCacheNt>(vb, rb(r));
   }
   stage_fft_twiddles<T, NT, NX2, R>(tw_h, tw_of(pool, NX2));
   stage_twiddles<T, NT>(tw_n, tw_of(pool, 2 * NX2), NX2);
@@ -447,6 +450,20 @@ __device__ __forceinline__ void RowPass<T, NX2, RB, R, BX, G, PP>::run(const Ins
       q[r] = (zk + zm) - cmul(cmuli(zk - zm), tw_n[k]);
     }
   };
+  // P = -i kx X1 (field 0), Q = X2 (field 1)
+  const T tkx = (T)d.tkx;
+  const BufField<cx<T>> gpm(d.pm), gqm(d.qm);
+  auto store_out = [&](const cx<T>(&q)[R], int f) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (f == 0) {
+        const T kx = tkx * (T)(j + r * TR);
+        gpm.template st<kCP>(vb, rb(r), cx<T>{kx * q[r].im, -kx * q[r].re});
+      } else {
+        gqm.template st<kCP>(vb, rb(r), q[r]);
+      }
+    }
+  };
   if constexpr (GR == 2) {
     if constexpr (!(kDbl && kLastC2R == 1)) __syncthreads();
     fft_multi<NX2, R, -1, 2>(f2, j, lds, tw_h);
@@ -458,7 +475,11 @@ __device__ __forceinline__ void RowPass<T, NX2, RB, R, BX, G, PP>::run(const Ins
     __syncthreads();
 #pragma unroll
     for (int f = 0; f < 2; ++f) r2c_post(f2[f], f);
+#pragma unroll
+    for (int f = 0; f < 2; ++f) store_out(f2[f], f);
   } else {
+    // each field is stored as soon as it is done, so P's stores drain while
+    // Q is transformed
 #pragma unroll
     for (int f = 0; f < 2; ++f) {
       cx<T>(&q)[1][R] = *reinterpret_cast<cx<T>(*)[1][R]>(&f2[f]);
@@ -469,17 +490,8 @@ __device__ __forceinline__ void RowPass<T, NX2, RB, R, BX, G, PP>::run(const Ins
       for (int r = 0; r < R; ++r) lds.store_plain(0, j + r * TR, q[0][r]);
       __syncthreads();
       r2c_post(q[0], 0);
+      store_out(q[0], f);
     }
-  }
-
-  const T tkx = (T)d.tkx;
-  const BufField<cx<T>> gpm(d.pm), gqm(d.qm);
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int k = j + r * TR;
-    const T kx = tkx * (T)k;
-    gpm.template st<kCP>(vb, rb(r), cx<T>{kx * f2[0][r].im, -kx * f2[0][r].re});   // -i kx X1
-    gqm.template st<kCP>(vb, rb(r), f2[1][r]);
   }
 
   double sums[3] = {esum, gxs, gys};
@@ -553,18 +565,17 @@ __device__ __forceinline__ void RowPassPaired<T, NX2, RB, R, BX, G, PP>::run(con
   auto rb = [](int r) -> unsigned { return (unsigned)r * S * RB * ESZ; };   // column j + r*S
 
   // f[field][r]: r < H: A at column j + r*S; r >= H: B at column j + (r-H)*S
+  // field by field, as in RowPass
   cx<T> f3[3][R];
   {
-    const BufField<cx<T>> gum(d.um), gvm(d.vm), gwm(d.wm);
+    const BufField<cx<T>> gf[3] = {BufField<cx<T>>(d.um), BufField<cx<T>>(d.vm), BufField<cx<T>>(d.wm)};
 #pragma unroll
-    for (int r = 0; r < H; ++r) {
-      f3[0][r] = gum.template ld<kCacheNt>(va, rb(r));
-      f3[1][r] = gvm.template ld<kCacheNt>(va, rb(r));
-      f3[2][r] = gwm.template ld<kCacheNt>(va, rb(r));
-      f3[0][r + H] = gum.template ld<kCacheNt>(va + vbo, rb(r));
-      f3[1][r + H] = gvm.template ld<kCacheNt>(va + vbo, rb(r));
-      f3[2][r + H] = gwm.template ld<kCacheNt>(va + vbo, rb(r));
-    }
+    for (int f = 0; f < 3; ++f)
+#pragma unroll
+      for (int r = 0; r < H; ++r) {
+        f3[f][r] = gf[f].template ld<kCacheNt>(va, rb(r));
+        f3[f][r + H] = gf[f].template ld<kCacheNt>(va + vbo, rb(r));
+      }
   }
   stage_fft_twiddles<T, NT, N, R>(sm.tw, tw_of(pool, N));
 
