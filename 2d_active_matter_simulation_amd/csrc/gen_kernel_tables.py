@@ -39,9 +39,16 @@ def lds_col(ny, W, R, GF, GI, esz, pp=0):
 # 1 double-buffered LDS exchanges, 2 streaming column-pass stores, 4 streaming
 # row-pass stores, 8 streaming column-pass loads of P, Q, 16 integrating-factor
 # table at radix 16 (FP32), 32 paired rows (two rows as one full-length complex
-# transform in the row pass).  The row kernel reads bits 1, 4 and 32, the
-# column kernel 1, 2, 8 and 16.
-ROW_OPTS, COL_OPTS = 37, 27
+# transform in the row pass), 64 half-brick row blocks (a row block covers
+# by/2 rows of the by-high bricks).  The row kernel reads bits 1, 4, 32 and
+# 64, the column kernel 1, 2, 8 and 16.
+ROW_OPTS, COL_OPTS = 101, 27
+
+
+def row_rows(by, pp):
+    """Rows per row block of a schedule (half the brick height with option 64)."""
+    return by // 2 if pp & 64 else by
+
 # Schedules that also get a paired-rows variant (options | 32), per precision
 # and size: the default and the tuned single-run and batched choices.
 PAIRED_BASES = {
@@ -64,6 +71,15 @@ STORE_VARIANT_BASES = {
     1024: [(4, 2, 4, 3, 4, 2, 3, 2, 1), (4, 2, 8, 1, 8, 1, 1, 2, 0), (4, 4, 8, 1, 16, 1, 1, 4, 0)],
     2048: [(4, 2, 16, 1, 16, 1, 1, 2, 0), (4, 2, 16, 1, 16, 1, 1, 2, 1), (4, 4, 8, 1, 16, 1, 1, 4, 0)],
     4096: [(4, 4, 16, 1, 16, 1, 1, 2, 0)],
+}
+# FP32 schedules that also get half-brick row-block variants (options | 64):
+# the row pass with half the rows per block (radix-8 rows of 2 x 2048 points
+# in 512 lanes) under the column pass of the tuned 4 x 4 bricks.  Measured
+# slower than whole-brick row blocks at 1024^2 and 2048^2, single and batched,
+# so only 4096^2 has them.
+HALF_ROW_BASES = {
+    4096: [(4, 4, 8, 1, 16, 1, 1, 2, 8), (4, 4, 8, 1, 16, 1, 1, 2, 12), (4, 4, 16, 1, 16, 1, 1, 2, 12),
+           (4, 4, 8, 1, 16, 1, 1, 2, 44), (4, 2, 8, 1, 16, 1, 1, 2, 12)],
 }
 
 
@@ -133,6 +149,8 @@ def candidates(prec, n):
     for mix in MULTI.get(prec, []):   # the schedules of the mixed launches
         alts += [s for m, s in mix if m == n]
     if prec == 32:
+        for b in HALF_ROW_BASES.get(n, []):
+            alts.append(b[:8] + (b[8] | 64,))
         for b in STORE_VARIANT_BASES.get(n, []):
             opts = list(range(2, 16, 2)) + ([16 + o for o in range(0, 16, 2)] if b[4] == 16 else [])
             alts += [b[:8] + (b[8] | o,) for o in opts]
@@ -151,11 +169,14 @@ def feasible(prec, nx, ny, s):
     tr, tc = nx2 // rr, ny // cr
     if tr % bx or nx2 % bx or ny % by or bx % cw or tc % by:
         return False
-    if pp & 32 and (by % 2 or rr < 2):   # paired rows: pairs of brick rows, mirrored half of each lane
+    if pp & 64 and by % 2:   # half-brick row blocks: an even brick height
         return False
-    if by * tr > 512 or cw * tc > 512:
+    rows = row_rows(by, pp)
+    if pp & 32 and (rows % 2 or rr < 2):   # paired rows: pairs of block rows, mirrored half of each lane
         return False
-    if lds_row(nx2, by, rr, rg, esz, pp) > LDS_MAX or lds_col(ny, cw, cr, gf, gi, esz, pp) > LDS_MAX:
+    if rows * tr > 512 or cw * tc > 512:
+        return False
+    if lds_row(nx2, rows, rr, rg, esz, pp) > LDS_MAX or lds_col(ny, cw, cr, gf, gi, esz, pp) > LDS_MAX:
         return False
     return True
 
@@ -163,11 +184,12 @@ def feasible(prec, nx, ny, s):
 HDR = """// {what} schedule and launch tables (the kernels are instantiated in
 // kernels_{tag}_k*.hip, compiled in parallel).
 // Generated by gen_kernel_tables.py -- edit that script, not this file.
-// Row kernel:    (Nx/2, RB = brick height, R, BX = brick width, G, PP)
+// Row kernel:    (Nx/2, RB = rows per block (the brick height, half of it with PP 64), R, BX = brick width, G, PP)
 // Column kernel: (Ny, W = columns per block, R, BY, BX = brick height/width, GF, GI, PP)
 // Schedule:      (Nx/2, Ny, BY, BX, row R, row G, col R, col GF, col GI, col W, PP, default)
 // PP: option bits (1 double-buffered LDS, 2 / 4 streaming column / row stores,
-// 8 streaming column-pass P, Q loads, 16 integrating-factor table at radix 16)
+// 8 streaming column-pass P, Q loads, 16 integrating-factor table at radix 16,
+// 32 paired rows, 64 half-brick row blocks)
 #include "kernels_inst.hpp"
 
 namespace af {{
@@ -209,7 +231,7 @@ def gen(prec, outdir):
                     scheds.append((nx // 2, ny) + s + (0,))
     rows, cols = set(), set()
     for (nx2, ny, by, bx, rr, rg, cr, gf, gi, cw, pp, _) in scheds:
-        rows.add((nx2, by, rr, bx, rg, pp & ROW_OPTS))
+        rows.add((nx2, row_rows(by, pp), rr, bx, rg, pp & ROW_OPTS))
         cols.add((ny, cw, cr, by, bx, gf, gi, pp & COL_OPTS))
     rl = [f"    {{{a}, {b}, {r}, {c}, {b * (a // r)}, launch_row<T, {a}, {b}, {r}, {c}, {g}, {pp}>, {g}, {pp}}},"
           for (a, b, r, c, g, pp) in sorted(rows)]
@@ -286,7 +308,7 @@ def gen_multi(outdir):
                 by, bx, rr, rg, cr, gf, gi, cw, pp = s
                 assert feasible(prec, n, n, s) and s in candidates(prec, n), (n, s)
                 assert by * (n // 2 // rr) == 256 and cw * (n // cr) == 256, (n, s)
-                assert not pp & 32, "mixed launches call the unpaired row body"
+                assert not pp & 96, "mixed launches call the unpaired whole-brick row body"
                 cls.append("{" + ", ".join(map(str, (n // 2, n, by, bx, rr, rg, cr, gf, gi, cw, pp, 0))) + "}")
                 rows.append(f"RowPass<{T}, {n // 2}, {by}, {rr}, {bx}, {rg}, {pp & ROW_OPTS}>")
                 cols.append(f"ColCfgT<{T}, {n}, {cw}, {cr}, {by}, {bx}, {gf}, {gi}, {pp & COL_OPTS}>")

@@ -40,11 +40,11 @@ template <int I, class C, class... Rest>
 __device__ __forceinline__ void row_dispatch(const MultiArgs& a, int b, unsigned char* smem) {
   using T = typename C::Real;
   if constexpr (sizeof...(Rest) == 0) {
-    C::run((const InstDesc<T>*)a.descs[I], a.tiles[I], (const cx<T>*)a.pool[I], b,
+    C::run((const InstDesc<T>*)a.descs[I], a.tiles[I], (const cx<T>*)a.pool[I], b, a.nblocks[I],
            *reinterpret_cast<typename C::Smem*>(smem));
   } else {
     if (b < a.nblocks[I])
-      C::run((const InstDesc<T>*)a.descs[I], a.tiles[I], (const cx<T>*)a.pool[I], b,
+      C::run((const InstDesc<T>*)a.descs[I], a.tiles[I], (const cx<T>*)a.pool[I], b, a.nblocks[I],
              *reinterpret_cast<typename C::Smem*>(smem));
     else
       row_dispatch<I + 1, Rest...>(a, b - a.nblocks[I], smem);

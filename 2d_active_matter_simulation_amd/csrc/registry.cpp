@@ -129,8 +129,9 @@ bool select_schedule(int precision, int nx2, int ny, int index) {
 const RowCfg* find_row_cfg(int precision, const Schedule* s) {
   int n = 0;
   const RowCfg* c = precision == 64 ? row_cfgs_f64(&n) : row_cfgs_f32(&n);
+  const int rows = (s->pp & kOptHalfRows) ? s->by / 2 : s->by;   // rows per row block
   for (int i = 0; i < n; ++i)
-    if (c[i].nx2 == s->nx2 && c[i].rb == s->by && c[i].r == s->row_r && c[i].bx == s->bx && c[i].g == s->row_g &&
+    if (c[i].nx2 == s->nx2 && c[i].rb == rows && c[i].r == s->row_r && c[i].bx == s->bx && c[i].g == s->row_g &&
         c[i].pp == (s->pp & kRowOptions))
       return &c[i];
   return nullptr;

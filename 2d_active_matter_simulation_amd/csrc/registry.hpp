@@ -19,11 +19,12 @@ using IfacLaunch = hipError_t (*)(const void* desc, int nx2, int ny, int w, hipS
 // row-pass stores (otherwise device-scope write-through), 8 streaming loads of
 // P, Q in the column pass, 16 FP32 integrating factors from the per-run table
 // also at radix 16 (radix <= 8 always reads it), 32 paired rows in the row
-// pass (two rows as one full-length complex transform).  A row kernel is
-// instantiated for the bits it reads only, as is a column kernel.
+// pass (two rows as one full-length complex transform), 64 half-brick row
+// blocks (a row block covers half of a brick row: by/2 rows).  A row kernel
+// is instantiated for the bits it reads only, as is a column kernel.
 constexpr int kOptLdsDouble = 1, kOptColStream = 2, kOptRowStream = 4, kOptPqStream = 8, kOptIfTable = 16,
-              kOptPairedRows = 32;
-constexpr int kRowOptions = kOptLdsDouble | kOptRowStream | kOptPairedRows;
+              kOptPairedRows = 32, kOptHalfRows = 64;
+constexpr int kRowOptions = kOptLdsDouble | kOptRowStream | kOptPairedRows | kOptHalfRows;
 constexpr int kColOptions = kOptLdsDouble | kOptColStream | kOptPqStream | kOptIfTable;
 
 struct RowCfg {

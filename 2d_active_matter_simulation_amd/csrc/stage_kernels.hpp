@@ -259,18 +259,46 @@ __device__ __forceinline__ double lds_wave_sum(const double* red, int k) {
 }
 
 // ---------------------------------------------------------------------------
-// Row pass.  Block = RB rows (one brick row) x TR = NX2/R lanes per row.
+// Row pass.  Block = RB rows (one brick row, or half of one) x TR = NX2/R
+// lanes per row.
 // G = 3: the three C2R transforms share their LDS exchanges (and the two
 // R2C transforms theirs); G = 1: one field at a time (less LDS).
 // PP, option bits: 1 = two sets of LDS areas, one barrier per transform
 // exchange (more LDS); 4 = streaming stores of P, Q (otherwise device-scope
 // write-through stores); 32 = paired rows (RowPassPaired below: rows y and
 // y + RB/2 of the block as the real and imaginary parts of one full-length
-// complex transform).  U, V, W are read with streaming loads.
+// complex transform); 64 = half-brick row blocks (RowTiles below).  U, V, W
+// are read with streaming loads.
 // ---------------------------------------------------------------------------
-// RowPass<...>::run is the body of one block (tile `block` of the launch,
-// block size NT, LDS `sm`); row_pass is the kernel of one size class, and a
-// mixed launch (multi_pass.hpp) calls the bodies of several classes.
+// Row tiles of a launch.  Without option 64 a row tile is one brick row (the
+// brick height is RB); with it, bricks are 2 RB rows high and a row tile is
+// the upper or lower half of a brick row, so a block holds half the rows (half
+// the LDS, twice the blocks) while the column pass keeps its taller bricks.
+// The two halves share every 128-byte line of their brick row, so tiles 2b
+// and 2b + 1 go to workgroup ids on one XCD (xcd_tile_order) and the line is
+// fetched into one L2.
+template <int RB, int BX, int NX2, int PP>
+struct RowTiles {
+  static constexpr bool kHalf = (PP & 64) != 0;
+  static constexpr int BYB = kHalf ? 2 * RB : RB;   // brick height
+  __device__ __forceinline__ static int order(int block, int nblocks) {
+    if constexpr (kHalf) return xcd_tile_order<2>(block, nblocks);
+    else return block;
+  }
+  // element offset of row yy of row tile `tile` at column 0
+  __device__ __forceinline__ static unsigned base(int tile, int yy) {
+    const int brow = kHalf ? tile >> 1 : tile;
+    const int y0 = kHalf ? (tile & 1) * RB : 0;
+    return (unsigned)brow * (NX2 / BX) * (BYB * BX) + (unsigned)(y0 + yy) * BX;
+  }
+  // offset of column k from column 0 of the same row
+  __device__ __forceinline__ static unsigned col(int k) { return (unsigned)(k / BX) * (BYB * BX) + (k % BX); }
+};
+
+// RowPass<...>::run is the body of one block (block `block` of the `nblocks`
+// of its class in the launch, block size NT, LDS `sm`); row_pass is the kernel
+// of one size class, and a mixed launch (multi_pass.hpp) calls the bodies of
+// several classes.
 template <class T, int NX2, int RB, int R, int BX, int G, int PP>
 struct RowPass {
   using Real = T;
@@ -287,7 +315,7 @@ struct RowPass {
     double red[4 * ((NT + 63) / 64)];
   };
   __device__ static void run(const InstDesc<T>* __restrict__ descs, int tiles_per_inst,
-                             const cx<T>* __restrict__ pool, int block, Smem& sm);
+                             const cx<T>* __restrict__ pool, int block, int nblocks, Smem& sm);
 };
 
 template <class T, int NX2, int RB, int R, int BX, int G, int PP>
@@ -298,16 +326,17 @@ __global__ void __launch_bounds__(RB * (NX2 / R))
 row_pass(const InstDesc<T>* __restrict__ descs, int tiles_per_inst, const cx<T>* __restrict__ pool) {
   using P = std::conditional_t<(PP & 32) != 0, RowPassPaired<T, NX2, RB, R, BX, G, PP>, RowPass<T, NX2, RB, R, BX, G, PP>>;
   __shared__ typename P::Smem sm;
-  P::run(descs, tiles_per_inst, pool, blockIdx.x, sm);
+  P::run(descs, tiles_per_inst, pool, blockIdx.x, gridDim.x, sm);
 }
 
 template <class T, int NX2, int RB, int R, int BX, int G, int PP>
 __device__ __forceinline__ void RowPass<T, NX2, RB, R, BX, G, PP>::run(const InstDesc<T>* __restrict__ descs,
                                                                       int tiles_per_inst,
                                                                       const cx<T>* __restrict__ pool, int block,
-                                                                      Smem& sm) {
+                                                                      int nblocks, Smem& sm) {
   static_assert(NX2 % BX == 0 && TR % BX == 0, "row lanes must cover whole bricks");
   static_assert(G == 1 || G == 3, "group of 1 or 3 fields");
+  using Tiles = RowTiles<RB, BX, NX2, PP>;
   constexpr int GR = G == 3 ? 2 : 1;   // R2C group
   constexpr int kCP = (PP & 4) ? kCacheNt : kCacheSc1;
   cx<T>* const lds_mem = sm.lds_mem;
@@ -315,27 +344,24 @@ __device__ __forceinline__ void RowPass<T, NX2, RB, R, BX, G, PP>::run(const Ins
   cx<T>* const tw_n = sm.tw_n;
   double* const red = sm.red;
 
-  const int inst = block / tiles_per_inst;
-  const int tile = block - inst * tiles_per_inst;
+  const int bid = Tiles::order(block, nblocks);
+  const int inst = bid / tiles_per_inst;
+  const int tile = bid - inst * tiles_per_inst;
   const InstDesc<T>& d = descs[inst];
   if (d.nx == 0) return;   // an empty entry of an engine's descriptor array: the whole block exits
   // lane -> (row yy of the block, transform lane j); a wave still covers a
-  // contiguous range of whole bricks
+  // contiguous range of whole bricks (or half bricks)
   const int t = threadIdx.x;
   const int yy = t % RB;
   const int j = t / RB;
   View lds{lds_mem, yy};
 
-  // brick offset of (row tile, position k) for this lane's row (fields hold
-  // at most 2^23 elements, so 32-bit offsets from a uniform base suffice)
-  const unsigned tile_base = (unsigned)tile * (NX2 / BX) * (RB * BX) + yy * BX;
-  auto addr = [&](int k) -> unsigned { return tile_base + (unsigned)(k / BX) * (RB * BX) + (k % BX); };
-
-  // element j + r*TR of this lane's row: lane offset addr(j) plus r*TR*RB
-  // (TR is a whole number of bricks)
+  // element j + r*TR of this lane's row: lane offset base + col(j) plus
+  // r*TR*BYB (TR is a whole number of bricks); fields hold at most 2^23
+  // elements, so 32-bit offsets from a uniform base suffice
   constexpr unsigned ESZ = sizeof(cx<T>);
-  const unsigned vb = addr(j) * ESZ;
-  auto rb = [](int r) -> unsigned { return (unsigned)r * TR * RB * ESZ; };
+  const unsigned vb = (Tiles::base(tile, yy) + Tiles::col(j)) * ESZ;
+  auto rb = [](int r) -> unsigned { return (unsigned)r * TR * Tiles::BYB * ESZ; };
   // Field by field: loads complete in issue order, so the first field's
   // exchange (and with G = 1 its whole transform) waits for its own loads
   // only while the other two fields are still in flight.
@@ -535,21 +561,23 @@ struct RowPassPaired {
     double red[4 * ((NT + 63) / 64)];
   };
   __device__ static void run(const InstDesc<T>* __restrict__ descs, int tiles_per_inst,
-                             const cx<T>* __restrict__ pool, int block, Smem& sm);
+                             const cx<T>* __restrict__ pool, int block, int nblocks, Smem& sm);
 };
 
 template <class T, int NX2, int RB, int R, int BX, int G, int PP>
 __device__ __forceinline__ void RowPassPaired<T, NX2, RB, R, BX, G, PP>::run(const InstDesc<T>* __restrict__ descs,
                                                                             int tiles_per_inst,
                                                                             const cx<T>* __restrict__ pool, int block,
-                                                                            Smem& sm) {
+                                                                            int nblocks, Smem& sm) {
   static_assert(G == 1 || G == 3, "group of 1 or 3 fields");
   static_assert(S % BX == 0, "pair lanes cover whole bricks");
+  using Tiles = RowTiles<RB, BX, NX2, PP>;
   constexpr int GR = G == 3 ? 2 : 1;   // R2C group
   constexpr int H = R / 2;             // loaded / stored points per lane and row
   constexpr int kCP = (PP & 4) ? kCacheNt : kCacheSc1;
-  const int inst = block / tiles_per_inst;
-  const int tile = block - inst * tiles_per_inst;
+  const int bid = Tiles::order(block, nblocks);
+  const int inst = bid / tiles_per_inst;
+  const int tile = bid - inst * tiles_per_inst;
   const InstDesc<T>& d = descs[inst];
   if (d.nx == 0) return;   // an empty entry of an engine's descriptor array: the whole block exits
   const int t = threadIdx.x;
@@ -558,11 +586,9 @@ __device__ __forceinline__ void RowPassPaired<T, NX2, RB, R, BX, G, PP>::run(con
   View lds{sm.lds_mem, yy};
 
   // brick offsets as in RowPass; row b sits W2 rows below row a in every brick
-  const unsigned tile_base = (unsigned)tile * (NX2 / BX) * (RB * BX) + yy * BX;
-  auto addr = [&](int k) -> unsigned { return tile_base + (unsigned)(k / BX) * (RB * BX) + (k % BX); };
   constexpr unsigned ESZ = sizeof(cx<T>);
-  const unsigned va = addr(j) * ESZ, vbo = (unsigned)(W2 * BX) * ESZ;
-  auto rb = [](int r) -> unsigned { return (unsigned)r * S * RB * ESZ; };   // column j + r*S
+  const unsigned va = (Tiles::base(tile, yy) + Tiles::col(j)) * ESZ, vbo = (unsigned)(W2 * BX) * ESZ;
+  auto rb = [](int r) -> unsigned { return (unsigned)r * S * Tiles::BYB * ESZ; };   // column j + r*S
 
   // f[field][r]: r < H: A at column j + r*S; r >= H: B at column j + (r-H)*S
   // field by field, as in RowPass
