@@ -298,3 +298,33 @@ def test_rectangular_grids_in_one_engine(af, orc):
             assert rel_l2(we, wo) <= 1e-12, (p.nx, p.ny)
             assert np.allclose(me, mo, rtol=1e-11, atol=1e-11)
     eng.close()
+
+
+def test_engine_4096_runs_equal_single_runs_bitwise(af):
+    """Two 4096^2 FP32 runs and a 256^2 run in one engine: at 4096^2 the
+    engine takes the single-run table's schedule (no batched line), whose row
+    pass uses half-brick row tiles when the table names option 64 (the two
+    halves of a brick row in sibling blocks, ordered over the whole launch of
+    both runs); each 4096^2 run equals the same run stepped alone through the
+    single-run API, bitwise, mean velocity and energies included."""
+    N = 4096
+    kw = dict(lam=3.5, alpha=-0.8, beta=0.01, gamma0=-0.045, gamma2=2.5e-5)
+    p = af.params(N, dt=5e-5, precision=32, **kw)
+    ws = [af.ic_bandlimited(N, seed=s, amp=2.0, kmax=12.0) for s in (31, 32)]
+    small = af.params(256, dt=2e-4, precision=32, **kw)
+    eng = af.Engine(1 << 34)
+    slots = [eng.add(p, w, (0.05 * N * N, -0.02 * N * N)) for w in ws]
+    eng.add(small, af.ic_bandlimited(256, seed=33, amp=1.0))
+    eng.step(3)
+    for w0, s in zip(ws, slots):
+        we, me = eng.get_state(s)
+        ee = eng.energy(s, 3)
+        run = af.Run(p)
+        run.set_state(w0, (0.05 * N * N, -0.02 * N * N))
+        er = run.step(3, energy=True)
+        wr, mr = run.get_state()
+        run.close()
+        assert np.array_equal(we, wr) and np.array_equal(me, mr), s
+        assert np.array_equal(ee, er), s
+    eng.close()
+    print(f"\n4096^2 single-run schedule {af.active_schedule(32, N)}")
