@@ -74,12 +74,22 @@ STORE_VARIANT_BASES = {
 }
 # FP32 schedules that also get half-brick row-block variants (options | 64):
 # the row pass with half the rows per block (radix-8 rows of 2 x 2048 points
-# in 512 lanes) under the column pass of the tuned 4 x 4 bricks.  Measured
-# slower than whole-brick row blocks at 1024^2 and 2048^2, single and batched,
-# so only 4096^2 has them.
+# in 512 lanes) under the column pass of the tuned 4 x 4 bricks.  On four-row
+# bricks they measured slower than whole-brick row blocks at 1024^2 and
+# 2048^2, single and batched, so only 4096^2 has them there; the eight-row
+# bricks below always use them.
 HALF_ROW_BASES = {
     4096: [(4, 4, 8, 1, 16, 1, 1, 2, 8), (4, 4, 8, 1, 16, 1, 1, 2, 12), (4, 4, 16, 1, 16, 1, 1, 2, 12),
            (4, 4, 8, 1, 16, 1, 1, 2, 44), (4, 2, 8, 1, 16, 1, 1, 2, 12)],
+}
+# Eight-row bricks (128- or 256-byte bricks for the column pass) with
+# half-brick (four-row) row tiles: the engine's 2048^2 choice (2.5-3.3 %
+# faster than 4 x 4 bricks for batches of 4 and 8) and its neighbours; at
+# 256^2-1024^2 (single runs and the mixed ensemble) they tied or lost.
+TALL_BRICK_BASES = {
+    1024: [(8, 2, 4, 3, 4, 2, 1, 2, 15)],
+    2048: [(8, 2, 16, 1, 16, 1, 1, 2, 16), (8, 4, 8, 1, 16, 1, 1, 4, 14), (8, 4, 8, 1, 16, 1, 1, 4, 10),
+           (8, 4, 8, 1, 16, 1, 1, 4, 46)],
 }
 
 
@@ -149,7 +159,7 @@ def candidates(prec, n):
     for mix in MULTI.get(prec, []):   # the schedules of the mixed launches
         alts += [s for m, s in mix if m == n]
     if prec == 32:
-        for b in HALF_ROW_BASES.get(n, []):
+        for b in HALF_ROW_BASES.get(n, []) + TALL_BRICK_BASES.get(n, []):
             alts.append(b[:8] + (b[8] | 64,))
         for b in STORE_VARIANT_BASES.get(n, []):
             opts = list(range(2, 16, 2)) + ([16 + o for o in range(0, 16, 2)] if b[4] == 16 else [])
