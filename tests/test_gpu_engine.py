@@ -138,9 +138,10 @@ def test_engine_schedule_matches_single_run_to_rounding(af):
 
 
 def test_engine_batched_schedule_2048_against_oracle(af, orc):
-    """Two 2048^2 FP32 runs in one engine (the tuned batched schedule: 4x4
-    bricks, radix-16 columns, 4-column blocks, streaming options) against the
-    FP64 oracle after one step, each to FP32 rounding."""
+    """Two 2048^2 FP32 runs in one engine (the tuned batched schedule: 8x4
+    bricks with half-brick row tiles, radix-16 columns, 4-column blocks,
+    streaming options) against the FP64 oracle after one step, each to FP32
+    rounding."""
     N = 2048
     kw = dict(lam=3.5, alpha=-0.8, beta=0.01, gamma0=-0.045, gamma2=2.5e-5)
     p = af.params(N, dt=5e-5, precision=32, **kw)

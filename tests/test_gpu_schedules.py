@@ -46,9 +46,10 @@ def test_every_schedule_matches_oracle(af, orc, prec, n):
 def test_every_schedule_one_step_at_large_sizes(af, orc, prec, n):
     """Every instantiated schedule of the sizes the tuner chooses among for
     the benchmarks (radix 4/8/16, grouped or single-field exchanges, brick
-    shapes, cache options, paired rows): one step from an O(1) band-limited
-    state with a mean flow against one FP64 oracle step, FP64 to 1e-12 and
-    FP32 to 2e-6 (relative L2 on omega); energy and mean velocity alike."""
+    shapes, cache options, paired rows, half-brick row tiles): one step from
+    an O(1) band-limited state with a mean flow against one FP64 oracle step,
+    FP64 to 1e-12 and FP32 to 2e-6 (relative L2 on omega); energy and mean
+    velocity alike."""
     p = af.params(n, dt=1e-4 if n <= 1024 else 5e-5, precision=prec, **JW)
     w0 = af.ic_bandlimited(n, seed=11, amp=2.0, kmax=12.0)
     M0 = (0.05 * n * n, -0.02 * n * n)
