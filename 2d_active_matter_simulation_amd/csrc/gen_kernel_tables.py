@@ -40,9 +40,10 @@ def lds_col(ny, W, R, GF, GI, esz, pp=0):
 # row-pass stores, 8 streaming column-pass loads of P, Q, 16 integrating-factor
 # table at radix 16 (FP32), 32 paired rows (two rows as one full-length complex
 # transform in the row pass), 64 half-brick row blocks (a row block covers
-# by/2 rows of the by-high bricks).  The row kernel reads bits 1, 4, 32 and
-# 64, the column kernel 1, 2, 8 and 16.
-ROW_OPTS, COL_OPTS = 101, 27
+# by/2 rows of the by-high bricks), 128 P, Q in bricks of 2 by rows (the row
+# pass stores and the column pass loads them so).  The row kernel reads bits
+# 1, 4, 32, 64 and 128, the column kernel 1, 2, 8, 16 and 128.
+ROW_OPTS, COL_OPTS = 229, 155
 
 
 def row_rows(by, pp):
@@ -81,6 +82,14 @@ STORE_VARIANT_BASES = {
 HALF_ROW_BASES = {
     4096: [(4, 4, 8, 1, 16, 1, 1, 2, 8), (4, 4, 8, 1, 16, 1, 1, 2, 12), (4, 4, 16, 1, 16, 1, 1, 2, 12),
            (4, 4, 8, 1, 16, 1, 1, 2, 44), (4, 2, 8, 1, 16, 1, 1, 2, 12)],
+}
+# Schedules that also get a tall-P, Q variant (options | 128: P, Q in bricks
+# of 2 by rows, so the column pass reads them in longer contiguous runs): the
+# 4096^2 single-run choice, where it wins (column pass 138 against 147 us);
+# for the tuned 256^2-2048^2 single-run, batched and mixed-ensemble choices
+# it measured within noise of them, so they are not instantiated.
+TALL_PQ_BASES = {
+    4096: [(4, 4, 8, 1, 16, 1, 1, 2, 76)],
 }
 # Eight-row bricks (128- or 256-byte bricks for the column pass) with
 # half-brick (four-row) row tiles: the engine's 2048^2 choice (2.5-3.3 %
@@ -161,6 +170,8 @@ def candidates(prec, n):
     if prec == 32:
         for b in HALF_ROW_BASES.get(n, []) + TALL_BRICK_BASES.get(n, []):
             alts.append(b[:8] + (b[8] | 64,))
+        for b in TALL_PQ_BASES.get(n, []):
+            alts.append(b[:8] + (b[8] | 128,))
         for b in STORE_VARIANT_BASES.get(n, []):
             opts = list(range(2, 16, 2)) + ([16 + o for o in range(0, 16, 2)] if b[4] == 16 else [])
             alts += [b[:8] + (b[8] | o,) for o in opts]
@@ -181,6 +192,8 @@ def feasible(prec, nx, ny, s):
         return False
     if pp & 64 and by % 2:   # half-brick row blocks: an even brick height
         return False
+    if pp & 128 and (pp & 32 or tc % (2 * by) or ny % (2 * by)):   # tall P, Q bricks
+        return False
     rows = row_rows(by, pp)
     if pp & 32 and (rows % 2 or rr < 2):   # paired rows: pairs of block rows, mirrored half of each lane
         return False
@@ -199,7 +212,7 @@ HDR = """// {what} schedule and launch tables (the kernels are instantiated in
 // Schedule:      (Nx/2, Ny, BY, BX, row R, row G, col R, col GF, col GI, col W, PP, default)
 // PP: option bits (1 double-buffered LDS, 2 / 4 streaming column / row stores,
 // 8 streaming column-pass P, Q loads, 16 integrating-factor table at radix 16,
-// 32 paired rows, 64 half-brick row blocks)
+// 32 paired rows, 64 half-brick row blocks, 128 P, Q in bricks twice as tall)
 #include "kernels_inst.hpp"
 
 namespace af {{
@@ -318,7 +331,7 @@ def gen_multi(outdir):
                 by, bx, rr, rg, cr, gf, gi, cw, pp = s
                 assert feasible(prec, n, n, s) and s in candidates(prec, n), (n, s)
                 assert by * (n // 2 // rr) == 256 and cw * (n // cr) == 256, (n, s)
-                assert not pp & 96, "mixed launches call the unpaired whole-brick row body"
+                assert not pp & 224, "mixed launches call the unpaired whole-brick row body"
                 cls.append("{" + ", ".join(map(str, (n // 2, n, by, bx, rr, rg, cr, gf, gi, cw, pp, 0))) + "}")
                 rows.append(f"RowPass<{T}, {n // 2}, {by}, {rr}, {bx}, {rg}, {pp & ROW_OPTS}>")
                 cols.append(f"ColCfgT<{T}, {n}, {cw}, {cr}, {by}, {bx}, {gf}, {gi}, {pp & COL_OPTS}>")

@@ -20,12 +20,13 @@ using IfacLaunch = hipError_t (*)(const void* desc, int nx2, int ny, int w, hipS
 // P, Q in the column pass, 16 FP32 integrating factors from the per-run table
 // also at radix 16 (radix <= 8 always reads it), 32 paired rows in the row
 // pass (two rows as one full-length complex transform), 64 half-brick row
-// blocks (a row block covers half of a brick row: by/2 rows).  A row kernel
-// is instantiated for the bits it reads only, as is a column kernel.
+// blocks (a row block covers half of a brick row: by/2 rows), 128 P, Q in
+// bricks of 2 by rows (both passes).  A row kernel is instantiated for the
+// bits it reads only, as is a column kernel.
 constexpr int kOptLdsDouble = 1, kOptColStream = 2, kOptRowStream = 4, kOptPqStream = 8, kOptIfTable = 16,
-              kOptPairedRows = 32, kOptHalfRows = 64;
-constexpr int kRowOptions = kOptLdsDouble | kOptRowStream | kOptPairedRows | kOptHalfRows;
-constexpr int kColOptions = kOptLdsDouble | kOptColStream | kOptPqStream | kOptIfTable;
+              kOptPairedRows = 32, kOptHalfRows = 64, kOptTallPq = 128;
+constexpr int kRowOptions = kOptLdsDouble | kOptRowStream | kOptPairedRows | kOptHalfRows | kOptTallPq;
+constexpr int kColOptions = kOptLdsDouble | kOptColStream | kOptPqStream | kOptIfTable | kOptTallPq;
 
 struct RowCfg {
   int nx2, rb, r, bx, threads;

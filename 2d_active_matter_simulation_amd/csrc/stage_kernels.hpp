@@ -267,8 +267,8 @@ __device__ __forceinline__ double lds_wave_sum(const double* red, int k) {
 // exchange (more LDS); 4 = streaming stores of P, Q (otherwise device-scope
 // write-through stores); 32 = paired rows (RowPassPaired below: rows y and
 // y + RB/2 of the block as the real and imaginary parts of one full-length
-// complex transform); 64 = half-brick row blocks (RowTiles below).  U, V, W
-// are read with streaming loads.
+// complex transform); 64 = half-brick row blocks, 128 = P, Q stored in bricks
+// twice as tall (RowTiles below).  U, V, W are read with streaming loads.
 // ---------------------------------------------------------------------------
 // Row tiles of a launch.  Without option 64 a row tile is one brick row (the
 // brick height is RB); with it, bricks are 2 RB rows high and a row tile is
@@ -293,6 +293,14 @@ struct RowTiles {
   }
   // offset of column k from column 0 of the same row
   __device__ __forceinline__ static unsigned col(int k) { return (unsigned)(k / BX) * (BYB * BX) + (k % BX); }
+  // P, Q (written here, read by the column pass) live in bricks twice as tall
+  // with option 128: the same offsets for a brick height of BYQ
+  static constexpr int BYQ = (PP & 128) ? 2 * BYB : BYB;
+  __device__ __forceinline__ static unsigned base_pq(int tile, int yy) {
+    const unsigned y = (unsigned)(tile * RB + yy);   // the row (row tiles are RB rows)
+    return (y / BYQ) * (NX2 / BX) * (BYQ * BX) + (y % BYQ) * BX;
+  }
+  __device__ __forceinline__ static unsigned col_pq(int k) { return (unsigned)(k / BX) * (BYQ * BX) + (k % BX); }
 };
 
 // RowPass<...>::run is the body of one block (block `block` of the `nblocks`
@@ -362,6 +370,9 @@ __device__ __forceinline__ void RowPass<T, NX2, RB, R, BX, G, PP>::run(const Ins
   constexpr unsigned ESZ = sizeof(cx<T>);
   const unsigned vb = (Tiles::base(tile, yy) + Tiles::col(j)) * ESZ;
   auto rb = [](int r) -> unsigned { return (unsigned)r * TR * Tiles::BYB * ESZ; };
+  // the same for the P, Q layout
+  const unsigned vbq = (Tiles::base_pq(tile, yy) + Tiles::col_pq(j)) * ESZ;
+  auto rbq = [](int r) -> unsigned { return (unsigned)r * TR * Tiles::BYQ * ESZ; };
   // Field by field: loads complete in issue order, so the first field's
   // exchange (and with G = 1 its whole transform) waits for its own loads
   // only while the other two fields are still in flight.
@@ -484,9 +495,9 @@ __device__ __forceinline__ void RowPass<T, NX2, RB, R, BX, G, PP>::run(const Ins
     for (int r = 0; r < R; ++r) {
       if (f == 0) {
         const T kx = tkx * (T)(j + r * TR);
-        gpm.template st<kCP>(vb, rb(r), cx<T>{kx * q[r].im, -kx * q[r].re});
+        gpm.template st<kCP>(vbq, rbq(r), cx<T>{kx * q[r].im, -kx * q[r].re});
       } else {
-        gqm.template st<kCP>(vb, rb(r), q[r]);
+        gqm.template st<kCP>(vbq, rbq(r), q[r]);
       }
     }
   };
@@ -572,6 +583,7 @@ __device__ __forceinline__ void RowPassPaired<T, NX2, RB, R, BX, G, PP>::run(con
   static_assert(G == 1 || G == 3, "group of 1 or 3 fields");
   static_assert(S % BX == 0, "pair lanes cover whole bricks");
   using Tiles = RowTiles<RB, BX, NX2, PP>;
+  static_assert(Tiles::BYQ == Tiles::BYB, "paired rows store P, Q in the U, V, W bricks");
   constexpr int GR = G == 3 ? 2 : 1;   // R2C group
   constexpr int H = R / 2;             // loaded / stored points per lane and row
   constexpr int kCP = (PP & 4) ? kCacheNt : kCacheSc1;
@@ -746,7 +758,7 @@ __device__ __forceinline__ void RowPassPaired<T, NX2, RB, R, BX, G, PP>::run(con
 // (v_x, v_y, omega) transforms share LDS exchanges; 1: one at a time.
 // PP, option bits: 1 = double-buffered LDS exchanges; 2 = streaming stores;
 // 8 = streaming loads of P, Q; 16 = FP32 integrating factors from the table
-// at radix 16 too.
+// at radix 16 too; 128 = P, Q in bricks twice as tall as the other fields'.
 // ---------------------------------------------------------------------------
 // ColPass<...>::run is the body of one block (block `block` of `nblocks` of
 // the class in the launch); col_pass is the kernel of one size class.
@@ -820,6 +832,14 @@ __device__ __forceinline__ void ColPass<T, NY, W, R, BY, BX, GF, GI, PP, stage>:
   const unsigned vb = addr(j) * ESZ;
   const unsigned rstride = (unsigned)(TC / BY) * brow * ESZ;   // bytes, uniform
   auto ld = [&](const cx<T>* f, int r) -> cx<T> { return BufField<cx<T>>(f).ld(vb, (unsigned)r * rstride); };
+  // P, Q in bricks twice as tall with option 128 (written so by the row pass)
+  constexpr int BYQ = (PP & 128) ? 2 * BY : BY;
+  static_assert(TC % BYQ == 0, "column lanes span whole P, Q bricks");
+  const unsigned brow_q = (unsigned)nx2 * BYQ;
+  const unsigned vbq =
+      ((unsigned)(tile * W / BX) * (BYQ * BX) + (unsigned)(tile * W % BX) + c + (unsigned)(j / BYQ) * brow_q +
+       (unsigned)(j % BYQ) * BX) * ESZ;
+  const unsigned rstride_q = (unsigned)(TC / BYQ) * brow_q * ESZ;
   auto st = [&](cx<T>* f, int r, cx<T> v) { BufField<cx<T>>(f).template st<kCP>(vb, (unsigned)r * rstride, v); };
 
   const double dt = d.dt, dt2 = d.dt2, dt3 = d.dt3, dt6 = d.dt6;
@@ -848,8 +868,8 @@ __device__ __forceinline__ void ColPass<T, NY, W, R, BY, BX, GF, GI, PP, stage>:
     // transforms and no lane holds the block up before its first barrier.
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      pq[0][r] = BufField<cx<T>>(d.pm).template ld<kPqCP>(vb, (unsigned)r * rstride);
-      pq[1][r] = BufField<cx<T>>(d.qm).template ld<kPqCP>(vb, (unsigned)r * rstride);
+      pq[0][r] = BufField<cx<T>>(d.pm).template ld<kPqCP>(vbq, (unsigned)r * rstride_q);
+      pq[1][r] = BufField<cx<T>>(d.qm).template ld<kPqCP>(vbq, (unsigned)r * rstride_q);
     }
     if (tile == 0) {
       double ps[3] = {0.0, 0.0, 0.0};
