@@ -43,7 +43,7 @@ def lds_col(ny, W, R, GF, GI, esz, pp=0):
 # by/2 rows of the by-high bricks), 128 P, Q in bricks of 2 by rows (the row
 # pass stores and the column pass loads them so).  The row kernel reads bits
 # 1, 4, 32, 64 and 128, the column kernel 1, 2, 8, 16 and 128.
-ROW_OPTS, COL_OPTS = 229, 155
+ROW_OPTS, COL_OPTS = 485, 411
 
 
 def row_rows(by, pp):
@@ -90,6 +90,18 @@ HALF_ROW_BASES = {
 # it measured within noise of them, so they are not instantiated.
 TALL_PQ_BASES = {
     4096: [(4, 4, 8, 1, 16, 1, 1, 2, 76)],
+}
+# Per-field layouts (options | 64 | 256 on 8-row bricks): U, V, W (read by the
+# row pass) in bricks of the row tile's 4 rows, so a tile reads one contiguous
+# run, while P, Q, acc and omega (read by the column pass) sit in the 8-row
+# bricks.  The 1024^2 and 2048^2 single runs and the 512^2 class of the mixed
+# ensemble take them (1.8 %, 1.8 % and 1.0 % per step); at 256^2, 512^2 single
+# and 4096^2 (where the column pass's scattered U, V, W stores cost 50 us) they
+# lost.
+SPLIT_LAYOUT_BASES = {
+    512: [(8, 4, 8, 1, 8, 1, 1, 4, 322)],
+    1024: [(8, 2, 4, 3, 4, 2, 1, 2, 335), (8, 2, 4, 3, 4, 2, 1, 2, 323)],
+    2048: [(8, 2, 16, 1, 16, 1, 1, 2, 336), (8, 2, 16, 1, 16, 1, 1, 2, 320)],
 }
 # Eight-row bricks (128- or 256-byte bricks for the column pass) with
 # half-brick (four-row) row tiles: the engine's 2048^2 choice (2.5-3.3 %
@@ -172,6 +184,7 @@ def candidates(prec, n):
             alts.append(b[:8] + (b[8] | 64,))
         for b in TALL_PQ_BASES.get(n, []):
             alts.append(b[:8] + (b[8] | 128,))
+        alts += SPLIT_LAYOUT_BASES.get(n, [])
         for b in STORE_VARIANT_BASES.get(n, []):
             opts = list(range(2, 16, 2)) + ([16 + o for o in range(0, 16, 2)] if b[4] == 16 else [])
             alts += [b[:8] + (b[8] | o,) for o in opts]
@@ -193,6 +206,8 @@ def feasible(prec, nx, ny, s):
     if pp & 64 and by % 2:   # half-brick row blocks: an even brick height
         return False
     if pp & 128 and (pp & 32 or tc % (2 * by) or ny % (2 * by)):   # tall P, Q bricks
+        return False
+    if pp & 256 and (pp & 32 or not pp & 64):   # U, V, W in half-height bricks: half-brick tiles
         return False
     rows = row_rows(by, pp)
     if pp & 32 and (rows % 2 or rr < 2):   # paired rows: pairs of block rows, mirrored half of each lane
@@ -331,7 +346,7 @@ def gen_multi(outdir):
                 by, bx, rr, rg, cr, gf, gi, cw, pp = s
                 assert feasible(prec, n, n, s) and s in candidates(prec, n), (n, s)
                 assert by * (n // 2 // rr) == 256 and cw * (n // cr) == 256, (n, s)
-                assert not pp & 224, "mixed launches call the unpaired whole-brick row body"
+                assert not pp & 480, "mixed launches call the unpaired whole-brick row body"
                 cls.append("{" + ", ".join(map(str, (n // 2, n, by, bx, rr, rg, cr, gf, gi, cw, pp, 0))) + "}")
                 rows.append(f"RowPass<{T}, {n // 2}, {by}, {rr}, {bx}, {rg}, {pp & ROW_OPTS}>")
                 cols.append(f"ColCfgT<{T}, {n}, {cw}, {cr}, {by}, {bx}, {gf}, {gi}, {pp & COL_OPTS}>")

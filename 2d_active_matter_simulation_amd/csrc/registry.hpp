@@ -21,12 +21,14 @@ using IfacLaunch = hipError_t (*)(const void* desc, int nx2, int ny, int w, hipS
 // also at radix 16 (radix <= 8 always reads it), 32 paired rows in the row
 // pass (two rows as one full-length complex transform), 64 half-brick row
 // blocks (a row block covers half of a brick row: by/2 rows), 128 P, Q in
-// bricks of 2 by rows (both passes).  A row kernel is instantiated for the
-// bits it reads only, as is a column kernel.
+// bricks of 2 by rows (both passes), 256 (with 64) U, V, W in bricks of by/2
+// rows (both passes).  A row kernel is instantiated for the bits it reads
+// only, as is a column kernel.
 constexpr int kOptLdsDouble = 1, kOptColStream = 2, kOptRowStream = 4, kOptPqStream = 8, kOptIfTable = 16,
-              kOptPairedRows = 32, kOptHalfRows = 64, kOptTallPq = 128;
-constexpr int kRowOptions = kOptLdsDouble | kOptRowStream | kOptPairedRows | kOptHalfRows | kOptTallPq;
-constexpr int kColOptions = kOptLdsDouble | kOptColStream | kOptPqStream | kOptIfTable | kOptTallPq;
+              kOptPairedRows = 32, kOptHalfRows = 64, kOptTallPq = 128, kOptShortUvw = 256;
+constexpr int kRowOptions =
+    kOptLdsDouble | kOptRowStream | kOptPairedRows | kOptHalfRows | kOptTallPq | kOptShortUvw;
+constexpr int kColOptions = kOptLdsDouble | kOptColStream | kOptPqStream | kOptIfTable | kOptTallPq | kOptShortUvw;
 
 struct RowCfg {
   int nx2, rb, r, bx, threads;

@@ -268,7 +268,8 @@ __device__ __forceinline__ double lds_wave_sum(const double* red, int k) {
 // write-through stores); 32 = paired rows (RowPassPaired below: rows y and
 // y + RB/2 of the block as the real and imaginary parts of one full-length
 // complex transform); 64 = half-brick row blocks, 128 = P, Q stored in bricks
-// twice as tall (RowTiles below).  U, V, W are read with streaming loads.
+// twice as tall, 256 = U, V, W read from bricks of the row tile's height
+// (RowTiles below).  U, V, W are read with streaming loads.
 // ---------------------------------------------------------------------------
 // Row tiles of a launch.  Without option 64 a row tile is one brick row (the
 // brick height is RB); with it, bricks are 2 RB rows high and a row tile is
@@ -294,13 +295,19 @@ struct RowTiles {
   // offset of column k from column 0 of the same row
   __device__ __forceinline__ static unsigned col(int k) { return (unsigned)(k / BX) * (BYB * BX) + (k % BX); }
   // P, Q (written here, read by the column pass) live in bricks twice as tall
-  // with option 128: the same offsets for a brick height of BYQ
+  // with option 128, and U, V, W (read here, written by the column pass) with
+  // option 256 in bricks of the row tile's height, so that a tile reads one
+  // contiguous run: the offsets of row yy of row tile `tile` and of column k
+  // for a brick height H
   static constexpr int BYQ = (PP & 128) ? 2 * BYB : BYB;
-  __device__ __forceinline__ static unsigned base_pq(int tile, int yy) {
+  static constexpr int BYU = (PP & 256) ? RB : BYB;
+  template <int H>
+  __device__ __forceinline__ static unsigned base_h(int tile, int yy) {
     const unsigned y = (unsigned)(tile * RB + yy);   // the row (row tiles are RB rows)
-    return (y / BYQ) * (NX2 / BX) * (BYQ * BX) + (y % BYQ) * BX;
+    return (y / H) * (NX2 / BX) * (H * BX) + (y % H) * BX;
   }
-  __device__ __forceinline__ static unsigned col_pq(int k) { return (unsigned)(k / BX) * (BYQ * BX) + (k % BX); }
+  template <int H>
+  __device__ __forceinline__ static unsigned col_h(int k) { return (unsigned)(k / BX) * (H * BX) + (k % BX); }
 };
 
 // RowPass<...>::run is the body of one block (block `block` of the `nblocks`
@@ -368,10 +375,10 @@ __device__ __forceinline__ void RowPass<T, NX2, RB, R, BX, G, PP>::run(const Ins
   // r*TR*BYB (TR is a whole number of bricks); fields hold at most 2^23
   // elements, so 32-bit offsets from a uniform base suffice
   constexpr unsigned ESZ = sizeof(cx<T>);
-  const unsigned vb = (Tiles::base(tile, yy) + Tiles::col(j)) * ESZ;
-  auto rb = [](int r) -> unsigned { return (unsigned)r * TR * Tiles::BYB * ESZ; };
+  const unsigned vb = (Tiles::template base_h<Tiles::BYU>(tile, yy) + Tiles::template col_h<Tiles::BYU>(j)) * ESZ;
+  auto rb = [](int r) -> unsigned { return (unsigned)r * TR * Tiles::BYU * ESZ; };
   // the same for the P, Q layout
-  const unsigned vbq = (Tiles::base_pq(tile, yy) + Tiles::col_pq(j)) * ESZ;
+  const unsigned vbq = (Tiles::template base_h<Tiles::BYQ>(tile, yy) + Tiles::template col_h<Tiles::BYQ>(j)) * ESZ;
   auto rbq = [](int r) -> unsigned { return (unsigned)r * TR * Tiles::BYQ * ESZ; };
   // Field by field: loads complete in issue order, so the first field's
   // exchange (and with G = 1 its whole transform) waits for its own loads
@@ -583,7 +590,7 @@ __device__ __forceinline__ void RowPassPaired<T, NX2, RB, R, BX, G, PP>::run(con
   static_assert(G == 1 || G == 3, "group of 1 or 3 fields");
   static_assert(S % BX == 0, "pair lanes cover whole bricks");
   using Tiles = RowTiles<RB, BX, NX2, PP>;
-  static_assert(Tiles::BYQ == Tiles::BYB, "paired rows store P, Q in the U, V, W bricks");
+  static_assert(Tiles::BYQ == Tiles::BYB && Tiles::BYU == Tiles::BYB, "paired rows: one brick shape for all fields");
   constexpr int GR = G == 3 ? 2 : 1;   // R2C group
   constexpr int H = R / 2;             // loaded / stored points per lane and row
   constexpr int kCP = (PP & 4) ? kCacheNt : kCacheSc1;
@@ -758,7 +765,8 @@ __device__ __forceinline__ void RowPassPaired<T, NX2, RB, R, BX, G, PP>::run(con
 // (v_x, v_y, omega) transforms share LDS exchanges; 1: one at a time.
 // PP, option bits: 1 = double-buffered LDS exchanges; 2 = streaming stores;
 // 8 = streaming loads of P, Q; 16 = FP32 integrating factors from the table
-// at radix 16 too; 128 = P, Q in bricks twice as tall as the other fields'.
+// at radix 16 too; 128 = P, Q in bricks twice as tall as the other fields';
+// 256 = U, V, W in bricks half as tall.
 // ---------------------------------------------------------------------------
 // ColPass<...>::run is the body of one block (block `block` of `nblocks` of
 // the class in the launch); col_pass is the kernel of one size class.
@@ -840,6 +848,16 @@ __device__ __forceinline__ void ColPass<T, NY, W, R, BY, BX, GF, GI, PP, stage>:
       ((unsigned)(tile * W / BX) * (BYQ * BX) + (unsigned)(tile * W % BX) + c + (unsigned)(j / BYQ) * brow_q +
        (unsigned)(j % BYQ) * BX) * ESZ;
   const unsigned rstride_q = (unsigned)(TC / BYQ) * brow_q * ESZ;
+  // U, V, W in bricks of half the height with option 256 (the row pass's
+  // half-brick tiles then read contiguous brick rows)
+  constexpr int BYU = (PP & 256) ? BY / 2 : BY;
+  static_assert(TC % BYU == 0 && BYU >= 1, "column lanes span whole U, V, W bricks");
+  const unsigned brow_u = (unsigned)nx2 * BYU;
+  const unsigned vbu =
+      ((unsigned)(tile * W / BX) * (BYU * BX) + (unsigned)(tile * W % BX) + c + (unsigned)(j / BYU) * brow_u +
+       (unsigned)(j % BYU) * BX) * ESZ;
+  const unsigned rstride_u = (unsigned)(TC / BYU) * brow_u * ESZ;
+  auto st_u = [&](cx<T>* f, int r, cx<T> v) { BufField<cx<T>>(f).template st<kCP>(vbu, (unsigned)r * rstride_u, v); };
   auto st = [&](cx<T>* f, int r, cx<T> v) { BufField<cx<T>>(f).template st<kCP>(vb, (unsigned)r * rstride, v); };
 
   const double dt = d.dt, dt2 = d.dt2, dt3 = d.dt3, dt6 = d.dt6;
@@ -1074,9 +1092,9 @@ __device__ __forceinline__ void ColPass<T, NY, W, R, BY, BX, GF, GI, PP, stage>:
     fft_multi<NY, R, +1, 3>(q3, j, lds, tw_y);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      st(d.um, r, q3[0][r]);
-      st(d.vm, r, q3[1][r]);
-      st(d.wm, r, q3[2][r]);
+      st_u(d.um, r, q3[0][r]);
+      st_u(d.vm, r, q3[1][r]);
+      st_u(d.wm, r, q3[2][r]);
     }
   } else {
 #pragma unroll
@@ -1087,7 +1105,7 @@ __device__ __forceinline__ void ColPass<T, NY, W, R, BY, BX, GF, GI, PP, stage>:
       fft_multi<NY, R, +1, 1>(q, j, lds, tw_y);
       cx<T>* const dst = (f == 0) ? d.um : (f == 1) ? d.vm : d.wm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) st(dst, r, q[0][r]);
+      for (int r = 0; r < R; ++r) st_u(dst, r, q[0][r]);
     }
   }
 }

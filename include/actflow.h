@@ -143,9 +143,11 @@ typedef struct {
                              * factors from a per-run table also at radix 16; *
                              * 32 paired rows (two rows of a row block as one *
                              * full-length complex transform); 64 half-brick  *
-                             * row blocks (by/2 rows per row-pass block, FP32 *
-                             * schedules).  2, 4 and 16 act on FP32 runs only *
-                             * (FP64 stores take the default policy).         */
+                             * row blocks (by/2 rows per row-pass block);     *
+                             * 128 P, Q in bricks of 2*by rows; 256 (with 64) *
+                             * U, V, W in bricks of by/2 rows (64, 128, 256:  *
+                             * FP32 schedules).  2, 4 and 16 act on FP32 runs *
+                             * only (FP64 stores take the default policy).    */
 } af_schedule;
 int af_schedule_count(int precision, int nx, int ny);
 af_status af_schedule_get(int precision, int nx, int ny, int index, af_schedule* out);

@@ -89,20 +89,22 @@ def test_schedule_struct_fields(af):
     """af_schedule carries the column-block width (divides the brick width)
     and the option bits (LDS double buffering; streaming stores, FP32 only;
     paired rows, with an even number of rows per row block; half-brick row
-    blocks, FP32 only, with an even brick height; P, Q in taller bricks, not
-    with paired rows)."""
+    blocks, FP32 only, with an even brick height; P, Q in taller and U, V, W
+    in shorter bricks, not with paired rows)."""
     for prec in (32, 64):
         for sc in af.schedules(prec, 1024):
             by, bx, rr, rg, cr, gf, gi, cw, opts = sc
             assert bx % cw == 0 and rr in (4, 8, 16) and cr in (4, 8, 16)
-            assert 0 <= opts < 256 and (prec == 32 or opts & ~32 in (0, 1))
+            assert 0 <= opts < 512 and (prec == 32 or opts & ~32 in (0, 1))
             rows = by // 2 if opts & 64 else by
             assert not opts & 64 or by % 2 == 0
             assert not opts & 32 or rows % 2 == 0
             assert not (opts & 128 and opts & 32)
+            assert not opts & 256 or (opts & 64 and not opts & 32)
     assert any(sc[8] & 6 for sc in af.schedules(32, 1024))
     assert any(sc[8] & 32 for sc in af.schedules(32, 1024)) and any(sc[8] & 32 for sc in af.schedules(64, 1024))
     assert any(sc[8] & 64 for sc in af.schedules(32, 4096)) and any(sc[8] & 128 for sc in af.schedules(32, 4096))
+    assert any(sc[8] & 256 for sc in af.schedules(32, 1024))
 
 
 def test_autotune_merge_keeps_untuned_lines():
