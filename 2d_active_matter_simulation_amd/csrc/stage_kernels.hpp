@@ -275,9 +275,10 @@ __device__ __forceinline__ double lds_wave_sum(const double* red, int k) {
 // brick height is RB); with it, bricks are 2 RB rows high and a row tile is
 // the upper or lower half of a brick row, so a block holds half the rows (half
 // the LDS, twice the blocks) while the column pass keeps its taller bricks.
-// The two halves share every 128-byte line of their brick row, so tiles 2b
-// and 2b + 1 go to workgroup ids on one XCD (xcd_tile_order) and the line is
-// fetched into one L2.
+// The two halves share the 128-byte lines of their brick row in every field
+// stored in those bricks, so tiles 2b and 2b + 1 go to workgroup ids on one
+// XCD (xcd_tile_order) and a line is fetched into (and written back from)
+// one L2.
 template <int RB, int BX, int NX2, int PP>
 struct RowTiles {
   static constexpr bool kHalf = (PP & 64) != 0;
